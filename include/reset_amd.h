@@ -1,0 +1,293 @@
+/*
+ * reset_amd.h -- C ABI of the MI355X-native RESET hash set-intersection path.
+ *
+ * This is the drop-in boundary.  The reference exposes this path as C++
+ * functions, classes and CUDA kernels (no FFI layer); each entry point below
+ * names the reference interface it replaces.  The C++ mirror of those classes
+ * (HashTableBuilder, TCBaseline, SSSBaseline, IRBaseline, make_hash_table) is
+ * in reset_amd.hpp, built on these functions.
+ *
+ * Conventions
+ *  - Plain pointers and explicit int64 sizes; no torch or HIP types.
+ *  - Pointers named d_* are device (HBM) pointers on the current HIP device;
+ *    h_* are host pointers.  Outputs are caller-owned unless stated.
+ *  - rs_stream is a hipStream_t (NULL = the legacy default stream).  *_async
+ *    calls only enqueue work; other calls synchronize `stream` before return.
+ *  - Elements are int32 bit patterns (uint32 ids < 2^31 in practice); the
+ *    value -1 (0xFFFFFFFF) is the EMPTY slot marker and may not be an element,
+ *    as in the reference (TC.cu:21).
+ *  - Every function returns an rs_status; rs_last_error() gives the message of
+ *    the calling thread's last failure.  Nothing exits the process.
+ *  - Handles are not thread-safe: one host thread per handle at a time.
+ */
+#ifndef RESET_AMD_H
+#define RESET_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rs_status;
+#define RS_OK 0
+#define RS_EINVAL 1     /* bad argument or malformed input                */
+#define RS_ENOMEM 2     /* device allocation failed / over memory budget  */
+#define RS_EOVERFLOW 3  /* bounded probe or insert hit a full table       */
+#define RS_EDEVICE 4    /* HIP runtime error                              */
+#define RS_EBUSY 5      /* serving queue full; retry after polling        */
+
+typedef void* rs_stream;
+
+/* Probe/build layouts (TC.cuh:40-52). */
+#define RS_NORMAL 0        /* bucket = x & (L-1)                          */
+#define RS_HIERARCHICAL 1  /* bucket = (x & (M-1)) >> (log2 M - log2 L)   */
+/* Layout masks for rs_tables_build. */
+#define RS_BUILD_NORMAL 1
+#define RS_BUILD_HIERARCHICAL 2
+#define RS_BUILD_BOTH 3
+
+const char* rs_version(void);
+const char* rs_status_string(rs_status s);
+const char* rs_last_error(void);
+
+/* ------------------------------------------------------------------ a1, a2 */
+/* triangle-count/hashtable.cu:20-36 calculate_length (host evaluation). */
+int32_t rs_calculate_length(int32_t degree, float load_factor, int32_t bucket_size);
+
+/* triangle-count/hashtable.cu:39-51 calculate_hash_length kernel:
+ * d_lengths[i] = calculate_length(off[i+1]-off[i]); *h_max_length = max. */
+rs_status rs_calculate_lengths(const int32_t* d_offsets, int64_t num_sets, float load_factor,
+                               int32_t bucket_size, int32_t* d_lengths, int32_t* h_max_length,
+                               rs_stream stream);
+
+/* ------------------------------------------------------------- a3-a6, a13 */
+typedef struct rs_tables rs_tables;
+
+typedef struct rs_tables_desc {
+  int64_t num_sets;             /* n                                        */
+  int64_t num_elements;         /* E = offsets[n]                           */
+  int64_t bucket_num;           /* BN = sum of L over sets (int64)          */
+  int32_t max_length;           /* M = max L                                */
+  int32_t bucket_size;          /* b (slots per bucket)                     */
+  float load_factor;            /* alpha                                    */
+  int32_t layouts;              /* RS_BUILD_* mask of the built tables      */
+  int32_t* d_table_normal;      /* int32[BN*b], slot k of bucket j of table */
+  int32_t* d_table_hierarchical;/*   t at [toff[t] + j + k*BN]; NULL if not built */
+  int32_t* d_lengths;           /* int32[n]   (d_hash_length)               */
+  int64_t* d_table_offsets;     /* int64[n+1] (d_hash_tables_offset)        */
+  const int32_t* d_offsets;     /* borrowed CSR offsets int32[n+1]          */
+  const int32_t* d_elements;    /* borrowed CSR elements int32[E]           */
+  int64_t conflicts_normal;     /* failed slot attempts while inserting     */
+  int64_t conflicts_hierarchical;
+} rs_tables_desc;
+
+/* Replaces make_hash_table(float, int) (triangle-count/hashtable.cu:103-184)
+ * and HashTableBuilder::build_hash_tables (baseline/common/hash_table.cuh:39-42,
+ * hash_table.cu:69-192): per-set lengths, int64 offset scan, -1 fill and the
+ * atomic-CAS insert of every (set, element) in the chosen layouts.  The CSR
+ * arrays are borrowed and must outlive the handle.  Synchronizes `stream`.
+ * RS_EOVERFLOW if some table had no free slot (load_factor >= 1). */
+rs_status rs_tables_build(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                          float load_factor, int32_t bucket_size, int32_t layouts,
+                          rs_stream stream, rs_tables** out);
+/* Getters of HashTableBuilder (hash_table.cuh:75-80). */
+rs_status rs_tables_describe(const rs_tables* t, rs_tables_desc* out);
+/* Replaces ~HashTableBuilder / free_hash_tables (hash_table.cuh:22-72). */
+void rs_tables_free(rs_tables* t);
+/* Synchronizes `stream` and reports (then clears) device-side error bits set
+ * by kernels that used this handle: RS_EOVERFLOW for an exhausted probe. */
+rs_status rs_tables_status(rs_tables* t, rs_stream stream);
+
+/* Row expansion vertexs[e] = row of entry e (triangle-count/hashtable.cu:148-158,
+ * TC.cu:277-282, done on the host there; a device kernel here). */
+rs_status rs_expand_rows(const int32_t* d_offsets, int64_t num_sets, int64_t num_elements,
+                         int32_t* d_rows, rs_stream stream);
+
+/* ------------------------------------------------------------------------ a7 */
+/* Replaces launch_extract_hashtable_to_csr (baseline/common/utils.cu:148-175)
+ * and gpu_sort_csr_cols (TC.cu:107-126): writes every set's elements in the
+ * (bucket, slot) order of its `layout` table -> d_out[E] (probe order). */
+rs_status rs_tables_probe_order(const rs_tables* t, int32_t layout, int32_t* d_out,
+                                rs_stream stream);
+
+/* ------------------------------------------------------------------------ a9 */
+/* Triangle counting over a degree- or id-oriented CSR whose tables `t` were
+ * built over that same CSR.  Replaces dynamic_triangle_count (TC.cu:128-190)
+ * and TCBaseline::run_normal / run_hierarchical (baseline/tc/tc.cuh:29-37).
+ * d_probe_keys: the CSR elements in the order the probes read them (NULL =
+ * the CSR itself; rs_tables_probe_order output for the hierarchical layout). */
+typedef struct rs_tc_plan rs_tc_plan;
+/* Edge schedule: per-edge row ids and the int64 prefix of probe counts
+ * min(d+(u), d+(v)), which the tiled kernel splits into equal probe tiles
+ * (hub edges are split across waves).  Reusable across layouts. */
+rs_status rs_tc_plan_create(const rs_tables* t, rs_stream stream, rs_tc_plan** out);
+void rs_tc_plan_free(rs_tc_plan* p);
+/* Total probes (sum over edges of min(d+(u), d+(v))) and edges of the plan. */
+rs_status rs_tc_plan_work(const rs_tc_plan* p, int64_t* h_probes, int64_t* h_edges);
+/* Tiled wave64 count into *d_count (uint64, overwritten). */
+rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
+                            const int32_t* d_probe_keys, uint64_t* d_count, rs_stream stream);
+/* The plain form: one wave per edge, CHUNK-edge work stealing, like the
+ * reference kernel (TC.cu:141-183).  Same result. */
+rs_status rs_tc_count_plain_async(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
+                                  const int32_t* d_probe_keys, uint64_t* d_count,
+                                  rs_stream stream);
+/* Convenience: plan (cached in t), count, synchronize, *h_count. */
+rs_status rs_tc_count(rs_tables* t, int32_t layout, const int32_t* d_probe_keys,
+                      uint64_t* h_count, rs_stream stream);
+
+/* ----------------------------------------------------------------------- a10 */
+/* Set-similarity verification.  Replaces set_similarity_search_kernel
+ * (set-similarity-search/sss.cu:93-154) and SSSBaseline::run_* (baseline/sss/
+ * sss.cuh:26-35): for candidate e = (u, v), r = |S_u cap S_v| by probing the
+ * smaller set into the larger one's table; d_flags[e] = 1 iff
+ * q*r >= p*(|S_u|+|S_v|-r) and |S_u|+|S_v|-r > 0 (exact form of the
+ * reference's float `r / (float)(|u|+|v|-r) >= threshold`).  d_sizes[e] = r
+ * (required, int32[P], used as the accumulator).  d_flags may be NULL. */
+rs_status rs_sss_verify_async(const rs_tables* t, int32_t layout, const int32_t* d_probe_keys,
+                              const int32_t* d_cand_u, const int32_t* d_cand_v,
+                              int64_t num_candidates, int32_t p, int32_t q, int32_t* d_flags,
+                              int32_t* d_sizes, rs_stream stream);
+
+/* Prefix-filter self-join of a set collection at Jaccard >= p/q: global
+ * token order by ascending frequency, prefix candidates with the size filter,
+ * batched hash verification with the exact rule.  The result is the sorted
+ * list of pairs (i < j) as int64 keys (i << 32 | j), held by the handle. */
+typedef struct rs_sss_result rs_sss_result;
+rs_status rs_sss_selfjoin(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                          int32_t p, int32_t q, float load_factor, int32_t bucket_size,
+                          rs_stream stream, rs_sss_result** out);
+rs_status rs_sss_result_info(const rs_sss_result* r, int64_t* h_num_pairs,
+                             int64_t* h_num_candidates, const int64_t** d_pairs);
+/* Stage timings of the last self-join (ms): token order, candidates, verify. */
+rs_status rs_sss_result_timings(const rs_sss_result* r, float* h_ms3);
+void rs_sss_result_free(rs_sss_result* r);
+
+/* ----------------------------------------------------------------------- a11 */
+/* Conjunctive queries.  Replaces ir_kernel (information-retrieval/ir.cu:94-176)
+ * and IRBaseline::run_* (baseline/ir/ir.cuh:39-46).  Tables `t` are built over
+ * the inverted index (sets = terms, elements = doc ids).  Query q has terms
+ * d_query[d_query_offsets[q] .. d_query_offsets[q+1]) with term0 first (the
+ * shortest list, as the preprocessing orders them).  Its result region is
+ * d_result[d_result_offsets[q] ..) of |list(term0)| entries (ir.cu:324-330);
+ * the result keeps term0's order as read from d_term0_docs (NULL = the index
+ * elements, i.e. ascending doc ids).  d_result_count[q] = result size.
+ * d_workspace: int32 scratch of d_result_offsets[Q] entries. */
+rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d_term0_docs,
+                            const int32_t* d_query, const int32_t* d_query_offsets,
+                            int64_t num_queries, const int64_t* d_result_offsets,
+                            int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
+                            rs_stream stream);
+
+/* IR serving loop with continuous batching (no counterpart in the reference,
+ * which runs one static batch).  The index and its tables stay resident; a
+ * host thread admits submitted queries while their result memory fits
+ * `memory_budget_bytes` and at most `max_batch` queries ride in one launch,
+ * keeps up to `max_inflight` launches overlapping on their own streams, and
+ * completes queries as their launch finishes. */
+typedef struct rs_ir_server rs_ir_server;
+typedef struct rs_ir_completion {
+  int64_t query_id;
+  int32_t count;          /* result size, or -1 on error                     */
+  int32_t status;         /* RS_OK / error                                   */
+  double submit_ms;       /* host clock at submit, ms                        */
+  double done_ms;         /* host clock when the result was ready, ms        */
+} rs_ir_completion;
+rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
+                              int32_t max_batch, int32_t max_inflight, rs_ir_server** out);
+/* Enqueue one query (terms on the host, term0 first).  RS_EBUSY if the
+ * pending queue is full. */
+rs_status rs_ir_server_submit(rs_ir_server* s, int64_t query_id, const int32_t* h_terms,
+                              int32_t num_terms);
+/* Drain up to `cap` completions; *h_n receives how many were written. */
+rs_status rs_ir_server_poll(rs_ir_server* s, rs_ir_completion* h_out, int32_t cap, int32_t* h_n);
+/* Copy a completed query's result (ascending doc ids if the index elements
+ * are sorted) to h_docs (cap entries). */
+rs_status rs_ir_server_result(rs_ir_server* s, int64_t query_id, int32_t* h_docs, int64_t cap,
+                              int64_t* h_count);
+/* Block until every submitted query completed. */
+rs_status rs_ir_server_drain(rs_ir_server* s);
+/* Launches, queries and peak admitted bytes so far. */
+rs_status rs_ir_server_stats(const rs_ir_server* s, int64_t* h_launches, int64_t* h_queries,
+                             int64_t* h_peak_bytes);
+void rs_ir_server_free(rs_ir_server* s);
+
+/* ----------------------------------------------------------------------- a12 */
+/* Pairwise set intersection sizes for P pairs (A_i, B_i) given as two CSR
+ * families with int64 offsets.  Sets need not be sorted for the hash probe;
+ * the batched form's range chunking requires ascending sets (flag below). */
+typedef struct rs_pairs_bin {
+  float load_factor;   /* alpha of the per-pair table                      */
+  int32_t bucket_size; /* b                                                */
+  int32_t chunk;       /* max elements of the build side per LDS table     */
+  int32_t unroll;      /* probe-side elements per lane per pass (1,2,4)    */
+} rs_pairs_bin;
+#define RS_PAIRS_MAX_BINS 20
+typedef struct rs_pairs_opts {
+  int32_t num_bins;                 /* bins by floor(log2(|A|+|B|)), <= 20   */
+  rs_pairs_bin bins[RS_PAIRS_MAX_BINS];
+  int32_t waves_per_cu;             /* persistent grid occupancy            */
+} rs_pairs_opts;
+/* Defaults tuned for gfx950 (also the autotuner's starting point). */
+void rs_pairs_default_opts(rs_pairs_opts* o);
+
+/* The plain form of set_size/intersection.cu:128-181: for each pair, a
+ * workspace table of the larger set is filled with -1, built and probed by
+ * the smaller set, one launch sequence per pair.  d_workspace must hold
+ * bucket_size * calculate_length(max |B|) int32. */
+rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* d_a,
+                                         const int64_t* d_b_off, const int32_t* d_b,
+                                         const int64_t* h_a_off, const int64_t* h_b_off,
+                                         int64_t num_pairs, float load_factor,
+                                         int32_t bucket_size, int32_t* d_workspace,
+                                         int32_t* d_sizes, rs_stream stream);
+
+/* The batched form: one persistent, size-binned, work-stealing launch.  A
+ * plan bins the pairs (largest first) once per batch; the launch builds each
+ * pair's smaller set into an LDS table (range-chunked when it exceeds the
+ * bin's chunk, which needs ascending sets) and streams the larger set through
+ * it.  The plan + launch sequence contains no memset and can be replayed from
+ * a hipGraph (rs_pairs_graph_*). */
+typedef struct rs_pairs_plan rs_pairs_plan;
+rs_status rs_pairs_plan_create(int64_t max_pairs, const rs_pairs_opts* opts, rs_pairs_plan** out);
+void rs_pairs_plan_free(rs_pairs_plan* p);
+rs_status rs_pairs_intersect_batched_async(rs_pairs_plan* plan, const int64_t* d_a_off,
+                                           const int32_t* d_a, const int64_t* d_b_off,
+                                           const int32_t* d_b, int64_t num_pairs, int32_t* d_sizes,
+                                           rs_stream stream);
+/* Capture the batched sequence for these buffers into a hipGraph held by the
+ * plan, then replay it with rs_pairs_graph_launch. */
+rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,
+                                 const int64_t* d_b_off, const int32_t* d_b, int64_t num_pairs,
+                                 int32_t* d_sizes, rs_stream stream);
+rs_status rs_pairs_graph_launch(rs_pairs_plan* plan, rs_stream stream);
+/* Device status of the plan's last launches (RS_EOVERFLOW on a full table). */
+rs_status rs_pairs_plan_status(rs_pairs_plan* p, rs_stream stream);
+
+/* Autotuner: times candidate (alpha, b, chunk, unroll) settings per size bin
+ * on the given pairs (a sample of the workload) and writes the fastest into
+ * *opts; the analytic occupancy model of hyperparameter_solver/src/solver.py
+ * ranks the candidates it tries first. */
+rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_a, const int64_t* d_b_off,
+                            const int32_t* d_b, int64_t num_pairs, int32_t reps,
+                            rs_pairs_opts* opts, rs_stream stream);
+
+/* --------------------------------------------------------------- generators */
+/* Synthetic inputs (not on the measured path): counter-based, seeded,
+ * reproducible.  Pair sets: A_i = sorted(pi_i[0 .. |A_i|)),
+ * B_i = sorted(pi_i[|A_i|-k_i .. |A_i|-k_i+|B_i|)) for a keyed bijection pi_i
+ * of [0, 2^24), so |A_i cap B_i| = k_i exactly. */
+rs_status rs_gen_pair_sets_async(const int64_t* d_a_off, const int64_t* d_b_off,
+                                 const int32_t* d_k, int64_t num_pairs, uint64_t seed,
+                                 int32_t* d_a, int32_t* d_b, rs_stream stream);
+/* R-MAT edges (Chakrabarti et al. 2004) with integer quadrant thresholds
+ * a,b,c (of 2^32) and vertex labels scrambled by a keyed bijection of
+ * [0, 2^scale): d_edges[2*i], d_edges[2*i+1] for i < num_edges. */
+rs_status rs_gen_rmat_async(int32_t scale, int64_t num_edges, uint32_t a, uint32_t b, uint32_t c,
+                            uint64_t seed, int32_t* d_edges, rs_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RESET_AMD_H */

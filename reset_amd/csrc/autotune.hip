@@ -1,0 +1,121 @@
+// autotune.hip -- per-size-bin choice of (load factor, bucket size, chunk,
+// unroll) for the batched pair kernel, by timing candidates on a sample.
+//
+// Candidates are the settings whose LDS table (calculate_length(chunk) x b
+// slots) fits one wave's 2048-slot region.  They are tried in order of the
+// expected LDS reads per unsuccessful probe under uniform hashing,
+// 1 + (occupied fraction of a home bucket's slots) -- a linear-probing
+// occupancy estimate -- and each bin keeps the fastest measured one.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace rs {
+rs_status pairs_launch_ordered(const rs_pairs_opts* opts, const int64_t* d_a_off,
+                               const int32_t* d_a, const int64_t* d_b_off, const int32_t* d_b,
+                               const int32_t* d_order, int64_t count, int32_t* d_ctl,
+                               int32_t* d_sizes, hipStream_t st);
+int pairs_ctl_words();
+int pairs_ctl_status_index();
+}  // namespace rs
+
+using namespace rs;
+
+namespace {
+struct Cand {
+  rs_pairs_bin bin;
+  double est;
+};
+
+std::vector<Cand> candidates() {
+  const float alphas[] = {0.2f, 0.25f, 0.5f, 0.75f, 0.9f};
+  const int bs[] = {1, 2, 4, 5};
+  const int chunks[] = {255, 511, 1023};
+  const int unrolls[] = {4, 8};
+  std::vector<Cand> out;
+  for (float a : alphas)
+    for (int b : bs)
+      for (int c : chunks) {
+        const int32_t L = calc_length(c, a, b);
+        if (L <= 0 || (int64_t)L * b > 2048) continue;
+        // fraction of slots in use when the chunk is full
+        const double load = (double)c / ((double)L * b);
+        const double est = 1.0 + load * b / (1.0 + (b - 1) * (1.0 - load)) + 64.0 / c;
+        for (int u : unrolls) out.push_back(Cand{rs_pairs_bin{a, b, c, u}, est});
+      }
+  std::stable_sort(out.begin(), out.end(), [](const Cand& x, const Cand& y) { return x.est < y.est; });
+  if (out.size() > 16) out.resize(16);
+  return out;
+}
+}  // namespace
+
+extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_a,
+                                       const int64_t* d_b_off, const int32_t* d_b,
+                                       int64_t num_pairs, int32_t reps, rs_pairs_opts* opts,
+                                       rs_stream stream) {
+  RS_REQUIRE(d_a_off && d_b_off && opts && num_pairs > 0 && reps > 0, "rs_pairs_autotune: bad args");
+  hipStream_t st = as_stream(stream);
+  std::vector<int64_t> ha(num_pairs + 1), hb(num_pairs + 1);
+  RS_HIP(hipMemcpyAsync(ha.data(), d_a_off, sizeof(int64_t) * ha.size(), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(hb.data(), d_b_off, sizeof(int64_t) * hb.size(), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  const int nb = opts->num_bins;
+  std::vector<std::vector<int32_t>> ids(nb);
+  for (int64_t i = 0; i < num_pairs; ++i) {
+    const int64_t tot = (ha[i + 1] - ha[i]) + (hb[i + 1] - hb[i]);
+    int bin = tot <= 1 ? 0 : 63 - __builtin_clzll((unsigned long long)tot);
+    bin = bin >= nb ? nb - 1 : bin;
+    ids[bin].push_back((int32_t)i);
+  }
+  DevBuf order, ctl, sizes;
+  RS_TRY(order.alloc(sizeof(int32_t) * (size_t)num_pairs));
+  RS_TRY(ctl.alloc(sizeof(int32_t) * (size_t)pairs_ctl_words()));
+  RS_TRY(sizes.alloc(sizeof(int32_t) * (size_t)num_pairs));
+  RS_HIP(hipMemsetAsync(ctl.p, 0, sizeof(int32_t) * (size_t)pairs_ctl_words(), st));
+  hipEvent_t e0, e1;
+  RS_HIP(hipEventCreate(&e0));
+  RS_HIP(hipEventCreate(&e1));
+  const auto cands = candidates();
+  rs_status result = RS_OK;
+  for (int bin = 0; bin < nb && result == RS_OK; ++bin) {
+    if (ids[bin].size() < 16) continue;
+    const int64_t cnt = (int64_t)ids[bin].size();
+    if (hipMemcpyAsync(order.p, ids[bin].data(), sizeof(int32_t) * (size_t)cnt,
+                       hipMemcpyHostToDevice, st) != hipSuccess) {
+      result = RS_EDEVICE;
+      break;
+    }
+    float best = 1e30f;
+    rs_pairs_bin best_bin = opts->bins[bin];
+    for (const Cand& c : cands) {
+      rs_pairs_opts trial = *opts;
+      for (int k = 0; k < nb; ++k) trial.bins[k] = c.bin;
+      float ms_min = 1e30f;
+      for (int r = 0; r <= reps && result == RS_OK; ++r) {  // r == 0 warms up
+        (void)hipEventRecord(e0, st);
+        result = pairs_launch_ordered(&trial, d_a_off, d_a, d_b_off, d_b, order.as<int32_t>(), cnt,
+                                      ctl.as<int32_t>(), sizes.as<int32_t>(), st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) ms_min = std::min(ms_min, ms);
+      }
+      if (ms_min < best) {
+        best = ms_min;
+        best_bin = c.bin;
+      }
+    }
+    opts->bins[bin] = best_bin;
+  }
+  int32_t err = 0;
+  (void)hipMemcpy(&err, ctl.as<int32_t>() + pairs_ctl_status_index(), sizeof(err), hipMemcpyDeviceToHost);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (result == RS_OK && err) {
+    set_error("rs_pairs_autotune: a candidate overflowed its table");
+    result = RS_EOVERFLOW;
+  }
+  return result;
+}

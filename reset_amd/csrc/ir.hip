@@ -1,0 +1,594 @@
+// ir.hip -- conjunctive queries over an inverted index (information-retrieval/
+// ir.cu:94-176) and a continuous-batching serving loop around them.
+//
+// Query kernel: term0's list (the shortest: queries are ordered by document
+// frequency) is cut into segments of kIrSeg entries; each wave filters one
+// segment through every later term's table, compacting survivors in place
+// with a 64-bit ballot (stable, so survivors keep term0's order).  Because
+// intersection distributes over the segments, a 2M-entry term0 list is spread
+// over hundreds of waves instead of one.  A second pass concatenates each
+// query's segment survivors into its result region (ir.cu:324-330 sizing).
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "tables.hpp"
+
+namespace rs {
+
+constexpr int kIrBlock = 256;
+constexpr int64_t kIrSeg = 4096;
+
+// per query: number of segments of term0's list (0 when it has no term or an
+// empty term0 list); validates term ids.
+__global__ void ir_seg_count_kernel(const int32_t* __restrict__ idx_off, int64_t nterms,
+                                    const int32_t* __restrict__ query,
+                                    const int32_t* __restrict__ qoff, int64_t Q,
+                                    const int64_t* __restrict__ roff, int32_t* __restrict__ nseg,
+                                    int32_t* status) {
+  bool bad = false;
+  for (int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < Q;
+       qi += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = qoff[qi], e = qoff[qi + 1];
+    int32_t ns = 0;
+    bool ok = e >= s;
+    for (int32_t i = s; ok && i < e; ++i) ok = query[i] >= 0 && query[i] < nterms;
+    if (ok && e > s) {
+      const int32_t t0 = query[s];
+      const int64_t n0 = idx_off[t0 + 1] - idx_off[t0];
+      ok = roff[qi + 1] - roff[qi] == n0;
+      ns = ok ? (int32_t)((n0 + kIrSeg - 1) / kIrSeg) : 0;
+    }
+    bad |= !ok;
+    nseg[qi] = ns;
+  }
+  if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
+}
+
+template <int LAYOUT>
+__global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
+    const int32_t* __restrict__ idx_off, const int32_t* term0_docs,
+    const int32_t* __restrict__ toff32_unused, const int32_t* __restrict__ Ls,
+    const int64_t* __restrict__ toff, const int32_t* __restrict__ T, int64_t BN, int32_t b,
+    uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff, int64_t Q,
+    const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t total_segs,
+    int32_t* ws, int32_t* __restrict__ seg_count, int32_t* __restrict__ status) {
+  (void)toff32_unused;
+  const int lane = __lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  bool bad = false;
+  for (int64_t g = wave; g < total_segs; g += nwaves) {
+    // query owning segment g: largest qi with segoff[qi] <= g
+    int64_t lo = 0, hi = Q;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (segoff[mid] <= g) lo = mid; else hi = mid;
+    }
+    const int64_t qi = lo;
+    const int64_t sidx = g - segoff[qi];
+    const int32_t qs = qoff[qi], qe = qoff[qi + 1];
+    const int32_t t0 = query[qs];
+    const int64_t n0 = idx_off[t0 + 1] - idx_off[t0];
+    const int64_t s0 = sidx * kIrSeg;
+    const int32_t len = (int32_t)(n0 - s0 < kIrSeg ? n0 - s0 : kIrSeg);
+    const int32_t* src = term0_docs + idx_off[t0] + s0;
+    int32_t* dst = ws + roff[qi] + s0;
+    int32_t cur = len;
+    if (qe - qs == 1) {
+      for (int32_t j = lane; j < len; j += kWave) dst[j] = src[j];
+    }
+    for (int32_t i = qs + 1; i < qe && cur > 0; ++i) {
+      const int32_t term = query[i];
+      const int32_t* Tt = T + toff[term];
+      const uint32_t L = (uint32_t)Ls[term];
+      int32_t w = 0;
+      for (int32_t j0 = 0; j0 < cur; j0 += kWave) {
+        const int32_t j = j0 + lane;
+        int32_t key = kEmpty;
+        bool hit = false;
+        if (j < cur) {
+          key = src[j];
+          const int r = probe(Tt, BN, home_bucket<LAYOUT>((uint32_t)key, L, M), L, b, key);
+          hit = r == 1;
+          bad |= r < 0;
+        }
+        const uint64_t m = __ballot(hit);
+        // ir.cu:151-156: stable in-place compaction (write position <= read)
+        if (hit) dst[w + __popcll(m & lanemask_lt())] = key;
+        w += __popcll(m);
+      }
+      cur = w;
+      src = dst;
+    }
+    if (lane == 0) seg_count[g] = cur;
+  }
+  if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
+}
+
+// One wave per query: concatenate the survivors of its segments into
+// result[roff[q] ..] and write count[q].
+__global__ void __launch_bounds__(kIrBlock) ir_gather_kernel(
+    const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t Q,
+    const int32_t* __restrict__ ws, const int32_t* __restrict__ seg_count,
+    int32_t* __restrict__ result, int32_t* __restrict__ count) {
+  const int lane = __lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t qi = wave; qi < Q; qi += nwaves) {
+    const int64_t g0 = segoff[qi], g1 = segoff[qi + 1];
+    int64_t w = roff[qi];
+    for (int64_t g = g0; g < g1; ++g) {
+      const int32_t c = seg_count[g];
+      const int32_t* src = ws + roff[qi] + (g - g0) * kIrSeg;
+      for (int32_t j = lane; j < c; j += kWave) result[w + j] = src[j];
+      w += c;
+    }
+    if (lane == 0) count[qi] = (int32_t)(w - roff[qi]);
+  }
+}
+
+// Workspace-carrying enqueue used by the C entry point and by the server.
+struct IrScratch {
+  DevBuf nseg, segoff, seg_count, tmp;
+  int64_t total_segs = 0;
+};
+
+static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* term0_docs,
+                            const int32_t* query, const int32_t* qoff, int64_t Q,
+                            const int64_t* roff, int32_t* result, int32_t* count, int32_t* ws,
+                            IrScratch& sc, hipStream_t st, bool sync_total, int64_t known_segs) {
+  const int32_t* T = nullptr;
+  RS_TRY(require_layout(t, layout, &T));
+  if (Q == 0) return RS_OK;
+  if (sc.nseg.bytes < sizeof(int32_t) * (size_t)Q) RS_TRY(sc.nseg.alloc(sizeof(int32_t) * (size_t)Q));
+  if (sc.segoff.bytes < sizeof(int64_t) * (size_t)(Q + 1))
+    RS_TRY(sc.segoff.alloc(sizeof(int64_t) * (size_t)(Q + 1)));
+  const int g = grid_for(Q, 256, 4);
+  ir_seg_count_kernel<<<g, 256, 0, st>>>(t->d_off, t->n, query, qoff, Q, roff,
+                                         sc.nseg.as<int32_t>(), t->d_status());
+  RS_TRY(exclusive_scan_i32_to_i64(sc.nseg.as<int32_t>(), Q, sc.segoff.as<int64_t>(), sc.tmp, st));
+  int64_t total = known_segs;
+  if (sync_total) {
+    RS_HIP(hipMemcpyAsync(&total, sc.segoff.as<int64_t>() + Q, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  sc.total_segs = total;
+  if (sc.seg_count.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
+    RS_TRY(sc.seg_count.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+  if (total > 0) {
+    const int grid = grid_for(total * kWave, kIrBlock, 8);
+    if (layout == RS_HIERARCHICAL)
+      ir_segment_kernel<RS_HIERARCHICAL><<<grid, kIrBlock, 0, st>>>(
+          t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
+          t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
+          sc.seg_count.as<int32_t>(), t->d_status());
+    else
+      ir_segment_kernel<RS_NORMAL><<<grid, kIrBlock, 0, st>>>(
+          t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
+          t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
+          sc.seg_count.as<int32_t>(), t->d_status());
+  }
+  ir_gather_kernel<<<grid_for(Q * kWave, kIrBlock, 8), kIrBlock, 0, st>>>(
+      roff, sc.segoff.as<int64_t>(), Q, ws, sc.seg_count.as<int32_t>(), result, count);
+  RS_HIP(hipGetLastError());
+  return RS_OK;
+}
+
+// ---------------------------------------------------------------- server
+// Packs each query's result densely after the batch: out_off[q] = prefix of
+// counts (single block, batches are <= a few thousand queries).
+__global__ void __launch_bounds__(256) ir_pack_kernel(const int64_t* __restrict__ roff,
+                                                      const int32_t* __restrict__ count, int64_t Q,
+                                                      const int32_t* __restrict__ result,
+                                                      int64_t* __restrict__ out_off,
+                                                      int32_t* __restrict__ dense) {
+  __shared__ int64_t carry_s;
+  __shared__ int64_t wsum[4];
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < Q; base += 256) {
+    const int64_t qi = base + threadIdx.x;
+    const int64_t c = qi < Q ? count[qi] : 0;
+    const int64_t inc = wave_inclusive_scan(c);
+    const int wid = threadIdx.x / kWave;
+    if (__lane_id() == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    int64_t before = carry_s;
+    for (int w = 0; w < wid; ++w) before += wsum[w];
+    if (qi < Q) out_off[qi] = before + inc - c;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out_off[Q] = carry_s;
+  __syncthreads();
+  const int lane = __lane_id();
+  for (int64_t qi = threadIdx.x / kWave; qi < Q; qi += 4) {
+    const int64_t s = roff[qi], d = out_off[qi];
+    const int32_t c = count[qi];
+    for (int32_t j = lane; j < c; j += kWave) dense[d + j] = result[s + j];
+  }
+}
+
+static double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace rs
+
+using namespace rs;
+
+struct IrPending {
+  int64_t id;
+  double submit_ms;
+  std::vector<int32_t> terms;
+  int64_t n0;  // |term0 list|
+};
+
+struct IrSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_counts = nullptr, ev_done = nullptr;
+  int state = 0;  // 0 free, 1 waiting for counts, 2 waiting for results
+  std::vector<IrPending> batch;
+  int64_t bytes = 0;
+  // host staging (pinned)
+  int32_t* h_query = nullptr;
+  int32_t* h_qoff = nullptr;
+  int64_t* h_roff = nullptr;
+  int32_t* h_count = nullptr;
+  int64_t* h_outoff = nullptr;
+  int32_t* h_dense = nullptr;
+  int64_t cap_q = 0, cap_terms = 0, cap_dense = 0;
+  // device
+  DevBuf d_query, d_qoff, d_roff, d_result, d_ws, d_count, d_outoff, d_dense;
+  IrScratch sc;
+};
+
+struct rs_ir_server {
+  const rs_tables* t = nullptr;
+  int32_t layout = 0;
+  int device = 0;
+  int64_t budget = 0;
+  int32_t max_batch = 0;
+  std::vector<int32_t> h_idx_off;  // host copy of the index offsets
+  std::vector<std::unique_ptr<IrSlot>> slots;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<IrPending> pending;
+  std::deque<rs_ir_completion> done;
+  std::unordered_map<int64_t, std::vector<int32_t>> results;
+  int64_t submitted = 0, completed = 0, launches = 0, admitted_bytes = 0, peak_bytes = 0;
+  bool stop = false;
+  rs_status worker_error = RS_OK;
+  std::thread worker;
+};
+
+template <typename T>
+static rs_status host_alloc(T*& p, int64_t n) {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  if (hipHostMalloc((void**)&p, sizeof(T) * (size_t)n, 0) != hipSuccess) {
+    p = nullptr;
+    set_error("ir server: hipHostMalloc of %lld entries failed", (long long)n);
+    return RS_ENOMEM;
+  }
+  return RS_OK;
+}
+
+// Grows a slot's pinned staging and device buffers (doubling) to hold a batch
+// of Q queries with `nterms` terms and `total_n0` term0 entries.
+static rs_status slot_reserve(IrSlot& s, int64_t Q, int64_t nterms, int64_t total_n0) {
+  if (Q + 1 > s.cap_q) {
+    const int64_t c = 2 * (Q + 1);
+    RS_TRY(host_alloc(s.h_qoff, c));
+    RS_TRY(host_alloc(s.h_roff, c));
+    RS_TRY(host_alloc(s.h_count, c));
+    RS_TRY(host_alloc(s.h_outoff, c));
+    RS_TRY(s.d_qoff.alloc(sizeof(int32_t) * (size_t)c));
+    RS_TRY(s.d_roff.alloc(sizeof(int64_t) * (size_t)c));
+    RS_TRY(s.d_count.alloc(sizeof(int32_t) * (size_t)c));
+    RS_TRY(s.d_outoff.alloc(sizeof(int64_t) * (size_t)c));
+    s.cap_q = c;
+  }
+  const int64_t nt = nterms > 0 ? nterms : 1;
+  if (nt > s.cap_terms) {
+    const int64_t c = 2 * nt;
+    RS_TRY(host_alloc(s.h_query, c));
+    RS_TRY(s.d_query.alloc(sizeof(int32_t) * (size_t)c));
+    s.cap_terms = c;
+  }
+  const int64_t need = total_n0 > 0 ? total_n0 : 1;
+  if (need > s.cap_dense) {
+    const int64_t c = 2 * need;
+    RS_TRY(host_alloc(s.h_dense, c));
+    RS_TRY(s.d_result.alloc(sizeof(int32_t) * (size_t)c));
+    RS_TRY(s.d_ws.alloc(sizeof(int32_t) * (size_t)c));
+    RS_TRY(s.d_dense.alloc(sizeof(int32_t) * (size_t)c));
+    s.cap_dense = c;
+  }
+  return RS_OK;
+}
+
+// Launch a batch on a slot: H2D query arrays, query kernels, pack, D2H counts.
+static rs_status slot_launch(rs_ir_server* S, IrSlot& s) {
+  const int64_t Q = (int64_t)s.batch.size();
+  int64_t nterms = 0, total = 0;
+  for (auto& p : s.batch) {
+    nterms += (int64_t)p.terms.size();
+    total += p.n0;
+  }
+  RS_TRY(slot_reserve(s, Q, nterms, total));
+  int64_t w = 0, r = 0;
+  for (int64_t i = 0; i < Q; ++i) {
+    s.h_qoff[i] = (int32_t)w;
+    s.h_roff[i] = r;
+    for (int32_t x : s.batch[i].terms) s.h_query[w++] = x;
+    r += s.batch[i].n0;
+  }
+  s.h_qoff[Q] = (int32_t)w;
+  s.h_roff[Q] = r;
+  hipStream_t st = s.stream;
+  RS_HIP(hipMemcpyAsync(s.d_query.p, s.h_query, sizeof(int32_t) * (size_t)(nterms > 0 ? nterms : 1),
+                        hipMemcpyHostToDevice, st));
+  RS_HIP(hipMemcpyAsync(s.d_qoff.p, s.h_qoff, sizeof(int32_t) * (size_t)(Q + 1),
+                        hipMemcpyHostToDevice, st));
+  RS_HIP(hipMemcpyAsync(s.d_roff.p, s.h_roff, sizeof(int64_t) * (size_t)(Q + 1),
+                        hipMemcpyHostToDevice, st));
+  int64_t segs = 0;
+  for (auto& p : s.batch) segs += (p.n0 + kIrSeg - 1) / kIrSeg;
+  RS_TRY(ir_enqueue(S->t, S->layout, S->t->d_elems, s.d_query.as<int32_t>(), s.d_qoff.as<int32_t>(),
+                    Q, s.d_roff.as<int64_t>(), s.d_result.as<int32_t>(), s.d_count.as<int32_t>(),
+                    s.d_ws.as<int32_t>(), s.sc, st, false, segs));
+  ir_pack_kernel<<<1, 256, 0, st>>>(s.d_roff.as<int64_t>(), s.d_count.as<int32_t>(), Q,
+                                    s.d_result.as<int32_t>(), s.d_outoff.as<int64_t>(),
+                                    s.d_dense.as<int32_t>());
+  RS_HIP(hipMemcpyAsync(s.h_count, s.d_count.p, sizeof(int32_t) * (size_t)Q, hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(s.h_outoff, s.d_outoff.p, sizeof(int64_t) * (size_t)(Q + 1),
+                        hipMemcpyDeviceToHost, st));
+  RS_HIP(hipEventRecord(s.ev_counts, st));
+  s.state = 1;
+  return RS_OK;
+}
+
+static void server_loop(rs_ir_server* S) {
+  (void)hipSetDevice(S->device);
+  while (true) {
+    bool progressed = false;
+    // 1. advance in-flight slots
+    for (auto& sp : S->slots) {
+      IrSlot& s = *sp;
+      if (s.state == 1 && hipEventQuery(s.ev_counts) == hipSuccess) {
+        const int64_t Q = (int64_t)s.batch.size();
+        const int64_t total = s.h_outoff[Q];
+        if (total > 0)
+          (void)hipMemcpyAsync(s.h_dense, s.d_dense.p, sizeof(int32_t) * (size_t)total,
+                               hipMemcpyDeviceToHost, s.stream);
+        (void)hipEventRecord(s.ev_done, s.stream);
+        s.state = 2;
+        progressed = true;
+      }
+      if (s.state == 2 && hipEventQuery(s.ev_done) == hipSuccess) {
+        const double t = now_ms();
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (size_t i = 0; i < s.batch.size(); ++i) {
+          const int32_t c = s.h_count[i];
+          const int64_t o = s.h_outoff[i];
+          S->results[s.batch[i].id] = std::vector<int32_t>(s.h_dense + o, s.h_dense + o + c);
+          S->done.push_back(rs_ir_completion{s.batch[i].id, c, RS_OK, s.batch[i].submit_ms, t});
+        }
+        S->completed += (int64_t)s.batch.size();
+        S->admitted_bytes -= s.bytes;
+        s.batch.clear();
+        s.bytes = 0;
+        s.state = 0;
+        progressed = true;
+        S->cv.notify_all();
+      }
+    }
+    // 2. admit pending queries into free slots (FIFO, memory-bounded)
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (S->stop) break;
+      for (auto& sp : S->slots) {
+        IrSlot& s = *sp;
+        if (s.state != 0 || S->pending.empty()) continue;
+        int64_t bytes = 0;
+        while (!S->pending.empty() && (int32_t)s.batch.size() < S->max_batch) {
+          const IrPending& p = S->pending.front();
+          const int64_t fb = 8 * p.n0 + 64 + 4 * (int64_t)p.terms.size();
+          if (S->admitted_bytes + bytes + fb > S->budget && !(s.batch.empty() && S->admitted_bytes == 0))
+            break;
+          bytes += fb;
+          s.batch.push_back(p);
+          S->pending.pop_front();
+        }
+        if (s.batch.empty()) continue;
+        s.bytes = bytes;
+        S->admitted_bytes += bytes;
+        S->peak_bytes = S->admitted_bytes > S->peak_bytes ? S->admitted_bytes : S->peak_bytes;
+        S->launches++;
+        lk.unlock();
+        rs_status st = slot_launch(S, s);
+        lk.lock();
+        if (st != RS_OK) {
+          const double t = now_ms();
+          for (auto& p : s.batch) S->done.push_back(rs_ir_completion{p.id, -1, st, p.submit_ms, t});
+          S->completed += (int64_t)s.batch.size();
+          S->admitted_bytes -= s.bytes;
+          s.batch.clear();
+          s.bytes = 0;
+          s.state = 0;
+          S->worker_error = st;
+          S->cv.notify_all();
+        }
+        progressed = true;
+      }
+      if (!progressed) {
+        bool busy = false;
+        for (auto& sp : S->slots) busy |= sp->state != 0;
+        if (busy)
+          S->cv.wait_for(lk, std::chrono::microseconds(20));
+        else
+          S->cv.wait_for(lk, std::chrono::milliseconds(2),
+                         [S] { return S->stop || !S->pending.empty(); });
+      }
+    }
+  }
+}
+
+extern "C" {
+
+rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d_term0_docs,
+                            const int32_t* d_query, const int32_t* d_query_offsets,
+                            int64_t num_queries, const int64_t* d_result_offsets,
+                            int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
+                            rs_stream stream) {
+  RS_REQUIRE(t, "rs_ir_query_async: null tables");
+  RS_REQUIRE(num_queries >= 0, "rs_ir_query_async: negative query count");
+  if (num_queries == 0) return RS_OK;
+  RS_REQUIRE(d_query_offsets && d_result_offsets && d_result_count && d_result && d_workspace,
+             "rs_ir_query_async: null argument");
+  hipStream_t st = as_stream(stream);
+  IrScratch sc;
+  RS_TRY(ir_enqueue(t, layout, d_term0_docs ? d_term0_docs : t->d_elems, d_query, d_query_offsets,
+                    num_queries, d_result_offsets, d_result, d_result_count, d_workspace, sc, st,
+                    true, 0));
+  RS_HIP(hipStreamSynchronize(st));  // scratch is released on return
+  return RS_OK;
+}
+
+rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
+                              int32_t max_batch, int32_t max_inflight, rs_ir_server** out) {
+  RS_REQUIRE(t && out, "rs_ir_server_create: null argument");
+  RS_REQUIRE(memory_budget_bytes > 0 && max_batch > 0 && max_inflight > 0 && max_inflight <= 16,
+             "rs_ir_server_create: bad limits");
+  const int32_t* T = nullptr;
+  RS_TRY(require_layout(t, layout, &T));
+  *out = nullptr;
+  auto* S = new rs_ir_server();
+  S->t = t;
+  S->layout = layout;
+  S->budget = memory_budget_bytes;
+  S->max_batch = max_batch;
+  (void)hipGetDevice(&S->device);
+  S->h_idx_off.resize((size_t)t->n + 1);
+  if (hipMemcpy(S->h_idx_off.data(), t->d_off, sizeof(int32_t) * (size_t)(t->n + 1),
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    delete S;
+    set_error("rs_ir_server_create: cannot read index offsets");
+    return RS_EDEVICE;
+  }
+  for (int i = 0; i < max_inflight; ++i) S->slots.push_back(std::make_unique<IrSlot>());
+  for (auto& sp : S->slots) {
+    IrSlot& s = *sp;
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s.ev_counts, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming) != hipSuccess) {
+      delete S;
+      set_error("rs_ir_server_create: stream/event creation failed");
+      return RS_EDEVICE;
+    }
+  }
+  S->worker = std::thread(server_loop, S);
+  *out = S;
+  return RS_OK;
+}
+
+rs_status rs_ir_server_submit(rs_ir_server* S, int64_t query_id, const int32_t* h_terms,
+                              int32_t num_terms) {
+  RS_REQUIRE(S && (h_terms || num_terms == 0) && num_terms >= 0, "rs_ir_server_submit: bad args");
+  IrPending p;
+  p.id = query_id;
+  p.submit_ms = now_ms();
+  p.terms.assign(h_terms, h_terms + num_terms);
+  for (int32_t x : p.terms)
+    RS_REQUIRE(x >= 0 && x < S->t->n, "rs_ir_server_submit: term id %d out of range", x);
+  p.n0 = num_terms > 0 ? S->h_idx_off[p.terms[0] + 1] - S->h_idx_off[p.terms[0]] : 0;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->pending.size() >= (size_t)1 << 22) {
+      set_error("rs_ir_server_submit: pending queue full");
+      return RS_EBUSY;
+    }
+    S->pending.push_back(std::move(p));
+    S->submitted++;
+  }
+  S->cv.notify_all();
+  return RS_OK;
+}
+
+rs_status rs_ir_server_poll(rs_ir_server* S, rs_ir_completion* h_out, int32_t cap, int32_t* h_n) {
+  RS_REQUIRE(S && h_n && (h_out || cap == 0), "rs_ir_server_poll: bad args");
+  std::lock_guard<std::mutex> lk(S->mu);
+  int32_t n = 0;
+  while (n < cap && !S->done.empty()) {
+    h_out[n++] = S->done.front();
+    S->done.pop_front();
+  }
+  *h_n = n;
+  return S->worker_error;
+}
+
+rs_status rs_ir_server_result(rs_ir_server* S, int64_t query_id, int32_t* h_docs, int64_t cap,
+                              int64_t* h_count) {
+  RS_REQUIRE(S && h_count, "rs_ir_server_result: bad args");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->results.find(query_id);
+  RS_REQUIRE(it != S->results.end(), "rs_ir_server_result: query %lld has no result",
+             (long long)query_id);
+  *h_count = (int64_t)it->second.size();
+  RS_REQUIRE(h_docs || it->second.empty() || cap == 0, "rs_ir_server_result: null buffer");
+  const int64_t n = *h_count < cap ? *h_count : cap;
+  if (n > 0) memcpy(h_docs, it->second.data(), sizeof(int32_t) * (size_t)n);
+  S->results.erase(it);
+  return RS_OK;
+}
+
+rs_status rs_ir_server_drain(rs_ir_server* S) {
+  RS_REQUIRE(S, "rs_ir_server_drain: null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->cv.wait(lk, [S] { return S->completed >= S->submitted; });
+  return S->worker_error;
+}
+
+rs_status rs_ir_server_stats(const rs_ir_server* S, int64_t* h_launches, int64_t* h_queries,
+                             int64_t* h_peak_bytes) {
+  RS_REQUIRE(S, "rs_ir_server_stats: null server");
+  auto* M = const_cast<rs_ir_server*>(S);
+  std::lock_guard<std::mutex> lk(M->mu);
+  if (h_launches) *h_launches = S->launches;
+  if (h_queries) *h_queries = S->completed;
+  if (h_peak_bytes) *h_peak_bytes = S->peak_bytes;
+  return RS_OK;
+}
+
+void rs_ir_server_free(rs_ir_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (auto& sp : S->slots) {
+    IrSlot& s = *sp;
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.ev_counts) (void)hipEventDestroy(s.ev_counts);
+    if (s.ev_done) (void)hipEventDestroy(s.ev_done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    for (void* p : {(void*)s.h_query, (void*)s.h_qoff, (void*)s.h_roff, (void*)s.h_count,
+                    (void*)s.h_outoff, (void*)s.h_dense})
+      if (p) (void)hipHostFree(p);
+  }
+  delete S;
+}
+
+}  // extern "C"

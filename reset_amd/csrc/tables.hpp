@@ -1,0 +1,33 @@
+// tables.hpp -- the per-set hash table family (host-side handle).
+#pragma once
+
+#include "common.hpp"
+
+struct rs_tc_plan;
+
+struct rs_tables {
+  int device = 0;
+  int64_t n = 0, E = 0, BN = 0;
+  int32_t M = 0, b = 0, layouts = 0;
+  float alpha = 0.f;
+  const int32_t* d_off = nullptr;
+  const int32_t* d_elems = nullptr;
+  rs::DevBuf lengths;    // int32[n]
+  rs::DevBuf toff;       // int64[n+1]
+  rs::DevBuf rows;       // int32[E]  row of each CSR entry
+  rs::DevBuf t_normal;   // int32[BN*b]
+  rs::DevBuf t_hier;     // int32[BN*b]
+  rs::DevBuf status;     // int32 error bits, then 2 x uint64 conflict counters
+  int64_t conflicts[2] = {0, 0};
+  rs_tc_plan* tc_plan = nullptr;  // cached by rs_tc_count
+
+  const int32_t* table(int32_t layout) const {
+    return layout == RS_HIERARCHICAL ? t_hier.as<int32_t>() : t_normal.as<int32_t>();
+  }
+  int32_t* d_status() const { return status.as<int32_t>(); }
+};
+
+namespace rs {
+// Checks that `layout` was built and returns the table or sets an error.
+rs_status require_layout(const rs_tables* t, int32_t layout, const int32_t** table);
+}  // namespace rs

@@ -1,0 +1,99 @@
+"""Synthetic workload inputs built on the device (seeded, reproducible).
+
+R-MAT graphs: HIP edge generator, then dedup / self-loop removal / orientation
+/ CSR with torch device ops.  Pair batches: per-pair sizes and overlaps drawn
+on the host with numpy, sets generated on the device (sorted, unique, exact
+overlap).  IR / set-similarity collections: Zipf inputs.
+These are input plumbing, not the measured path.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import api
+
+RMAT_A, RMAT_B, RMAT_C = 0.57, 0.19, 0.19
+
+
+def rmat_undirected(scale, edge_factor=16, seed=20261203, device="cuda"):
+    """Unique undirected edges (u < v, no self-loops) of an R-MAT graph:
+    int64 tensors (lo, hi), n = 2^scale."""
+    n = 1 << scale
+    m = edge_factor * n
+    e = api.gen_rmat_edges(scale, m, RMAT_A, RMAT_B, RMAT_C, seed=seed, device=device).long()
+    u, v = e[:, 0], e[:, 1]
+    del e
+    keep = u != v
+    u, v = u[keep], v[keep]
+    lo = torch.minimum(u, v)
+    hi = torch.maximum(u, v)
+    del u, v, keep
+    key = torch.unique(lo * n + hi)
+    return key // n, key % n, n
+
+
+def orient_csr(lo, hi, n, by="degree"):
+    """Acyclic orientation low-degree -> high-degree (ties by id), or by id
+    (u -> v for u < v, the reference preprocessing's rule); int32 CSR with
+    rows sorted ascending."""
+    if by == "degree":
+        deg = torch.bincount(torch.cat([lo, hi]), minlength=n)
+        fwd = (deg[lo] < deg[hi]) | ((deg[lo] == deg[hi]) & (lo < hi))
+        src = torch.where(fwd, lo, hi)
+        dst = torch.where(fwd, hi, lo)
+        del deg, fwd
+    else:
+        src, dst = lo, hi
+    key, _ = torch.sort(src * n + dst)
+    src = key // n
+    dst = (key % n).to(torch.int32)
+    off = torch.zeros(n + 1, dtype=torch.int64, device=lo.device)
+    off[1:] = torch.cumsum(torch.bincount(src, minlength=n), 0)
+    return off.to(torch.int32), dst
+
+
+def symmetric_csr(lo, hi, n):
+    """Both directions (the set-similarity CSR of preprocessor.py:161-184)."""
+    src = torch.cat([lo, hi])
+    dst = torch.cat([hi, lo])
+    key, _ = torch.sort(src * n + dst)
+    off = torch.zeros(n + 1, dtype=torch.int64, device=lo.device)
+    off[1:] = torch.cumsum(torch.bincount(key // n, minlength=n), 0)
+    return off.to(torch.int32), (key % n).to(torch.int32)
+
+
+def rmat_tc_csr(scale, edge_factor=16, seed=20261203, by="degree", device="cuda"):
+    lo, hi, n = rmat_undirected(scale, edge_factor, seed, device)
+    return orient_csr(lo, hi, n, by)
+
+
+# ---------------------------------------------------------------- pairs
+def pair_meta(num_pairs, seed=20261203, lo=16, hi=65536, overlap_max=0.5):
+    """Sizes log-uniform in [lo, hi] (floor(exp(U[ln lo, ln(hi+1))))) and
+    overlap k = floor(f * min(|A|, |B|)), f ~ U[0, overlap_max]."""
+    rng = np.random.default_rng(seed)
+    ln_lo, ln_hi = math.log(lo), math.log(hi + 1)
+    na = np.floor(np.exp(rng.uniform(ln_lo, ln_hi, num_pairs))).astype(np.int64)
+    nb = np.floor(np.exp(rng.uniform(ln_lo, ln_hi, num_pairs))).astype(np.int64)
+    na = np.clip(na, lo, hi)
+    nb = np.clip(nb, lo, hi)
+    f = rng.uniform(0.0, overlap_max, num_pairs)
+    k = np.floor(f * np.minimum(na, nb)).astype(np.int64)
+    return na, nb, k
+
+
+def make_pair_batch(num_pairs, seed=20261203, device="cuda", **kw):
+    """-> dict(a_off, a, b_off, b, k) on the device (int64 offsets, int32 sets)."""
+    na, nb, k = pair_meta(num_pairs, seed, **kw)
+    a_off = np.zeros(num_pairs + 1, np.int64)
+    b_off = np.zeros(num_pairs + 1, np.int64)
+    np.cumsum(na, out=a_off[1:])
+    np.cumsum(nb, out=b_off[1:])
+    a_off_d = torch.from_numpy(a_off).to(device)
+    b_off_d = torch.from_numpy(b_off).to(device)
+    k_d = torch.from_numpy(k.astype(np.int32)).to(device)
+    a, b = api.gen_pair_sets(a_off_d, b_off_d, k_d, seed)
+    return dict(a_off=a_off_d, a=a, b_off=b_off_d, b=b, k=k_d)

@@ -1,0 +1,147 @@
+"""GPU parity: batched and plain pair intersection, and the pair generator."""
+import numpy as np
+import pytest
+import torch
+
+import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    assert torch.cuda.is_available(), "GPU tests need the MI355X"
+    from reset_amd import api as A
+
+    A.load()
+    return A
+
+
+def _dev(a_off, a, b_off, b):
+    c = lambda x, dt: torch.from_numpy(np.ascontiguousarray(x)).to(dt).cuda()
+    return c(a_off, torch.int64), c(a, torch.int32), c(b_off, torch.int64), c(b, torch.int32)
+
+
+def test_generator_sets_sorted_unique_exact_overlap(api):
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(400, seed=3, lo=16, hi=9000)
+    a_off, a = batch["a_off"].cpu().numpy(), batch["a"].cpu().numpy()
+    b_off, b = batch["b_off"].cpu().numpy(), batch["b"].cpu().numpy()
+    k = batch["k"].cpu().numpy()
+    for i in range(400):
+        A = a[a_off[i]:a_off[i + 1]]
+        B = b[b_off[i]:b_off[i + 1]]
+        assert np.all(np.diff(A) > 0) and np.all(np.diff(B) > 0)
+        assert A.min() >= 0 and A.max() < 1 << 24 and B.max() < 1 << 24
+        assert len(np.intersect1d(A, B)) == k[i]
+    # deterministic: same seed, same bits
+    again = inputs.make_pair_batch(400, seed=3, lo=16, hi=9000)
+    assert torch.equal(again["a"], batch["a"]) and torch.equal(again["b"], batch["b"])
+
+
+def test_batched_equals_oracle_and_plain_numpy_sets(api, oracle_lib):
+    rng = np.random.default_rng(1)
+    a_off, a, b_off, b, k = G.make_pairs(rng, 600, 3000, 1 << 24)
+    d = _dev(a_off, a, b_off, b)
+    plan = api.PairsPlan(600)
+    got = plan.run(*d).cpu().numpy()
+    assert np.array_equal(got, k)
+    assert np.array_equal(got, oracle_lib.pairs_hash(a_off, a, b_off, b))
+    plain = api.pairs_intersect_plain(*d).cpu().numpy()
+    assert np.array_equal(plain, k)
+    plan.close()
+
+
+@pytest.mark.parametrize("sizes", [(0, 0), (0, 50), (1, 1), (511, 511), (512, 512), (513, 4000),
+                                   (1023, 70000), (2048, 2048), (65536, 65536), (5000, 131072)])
+def test_batched_chunk_boundaries(api, oracle_lib, sizes):
+    """Build sides at, below and above the LDS chunk (range chunking)."""
+    na, nb = sizes
+    rng = np.random.default_rng(na * 7 + nb)
+    universe = 1 << 24
+    k = min(na, nb) // 3
+    pool = rng.choice(universe, size=na + nb - k, replace=False)
+    A = np.sort(pool[:na])
+    B = np.sort(np.concatenate([pool[na - k:na], pool[na:]])) if nb else np.zeros(0, np.int64)
+    a_off = np.array([0, na], np.int64)
+    b_off = np.array([0, len(B)], np.int64)
+    d = _dev(a_off, A, b_off, B)
+    plan = api.PairsPlan(1)
+    assert plan.run(*d).cpu().tolist() == [k]
+    # swapped roles give the same size
+    d2 = _dev(b_off, B, a_off, A)
+    assert plan.run(*d2).cpu().tolist() == [k]
+    plan.close()
+
+
+def test_batched_config2_shape_sample(api, oracle_lib):
+    """A 20k-pair sample of the 1M-pair workload (log-uniform [16, 65536],
+    overlap U[0, 0.5]): batched == plain == k == oracle."""
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(20000, seed=20261203)
+    plan = api.PairsPlan(20000)
+    got = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"])
+    assert torch.equal(got, batch["k"])
+    sub = slice(0, 300)
+    a_off = batch["a_off"].cpu().numpy()
+    b_off = batch["b_off"].cpu().numpy()
+    a = batch["a"].cpu().numpy()
+    b = batch["b"].cpu().numpy()
+    want = oracle_lib.pairs_hash(a_off[:301], a, b_off[:301], b)
+    assert np.array_equal(got.cpu().numpy()[sub], want)
+    P = 300
+    plain = api.pairs_intersect_plain(batch["a_off"][:P + 1].contiguous(), batch["a"],
+                                      batch["b_off"][:P + 1].contiguous(), batch["b"])
+    assert np.array_equal(plain.cpu().numpy(), want)
+    plan.close()
+
+
+def test_graph_replays_match_eager(api):
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(5000, seed=11)
+    plan = api.PairsPlan(5000)
+    eager = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"]).clone()
+    s = torch.cuda.Stream()
+    sizes = torch.zeros(5000, dtype=torch.int32, device="cuda")
+    with torch.cuda.stream(s):
+        plan.capture(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes, s)
+    for rep in range(4):
+        sizes.fill_(-7)
+        torch.cuda.synchronize()
+        plan.replay(s)
+        s.synchronize()
+        assert torch.equal(sizes, eager), rep
+        _ = (batch["a"][:1000].float().sum())  # eager work between replays
+        torch.cuda.synchronize()
+    plan.status()
+    plan.close()
+
+
+@pytest.mark.parametrize("alpha,b,chunk,unroll", [(0.2, 5, 255, 8), (0.5, 2, 1000, 4), (0.9, 1, 1000, 2),
+                                                  (0.25, 2, 511, 1), (0.12, 4, 200, 8)])
+def test_batched_bin_parameters(api, alpha, b, chunk, unroll):
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(3000, seed=5, hi=20000)
+    opts = api.default_pairs_opts()
+    for i in range(opts.num_bins):
+        opts.bins[i].load_factor = alpha
+        opts.bins[i].bucket_size = b
+        opts.bins[i].chunk = chunk
+        opts.bins[i].unroll = unroll
+    plan = api.PairsPlan(3000, opts)
+    got = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"])
+    assert torch.equal(got, batch["k"])
+    plan.close()
+
+
+def test_bin_parameters_rejected_when_table_exceeds_lds(api):
+    from reset_amd.lib import ResetError
+
+    opts = api.default_pairs_opts()
+    opts.bins[3].chunk = 5000
+    with pytest.raises(ResetError):
+        api.PairsPlan(10, opts)
