@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X hash set-intersection path.
+
+Default workload (the headline metric, set-pair intersections/s on one GPU):
+1,000,000 set pairs, sizes log-uniform in [16, 65536], sorted unique uint32
+elements from a 2^24 universe, overlap fraction U[0, 0.5]; all pairs in one
+persistent size-binned batched launch per step.  Prints ONE JSON line.
+
+    python bench.py [--steps K] [--warmup W]            # pairs (default)
+    python bench.py --workload tc --scale 24            # R-MAT triangle counting
+
+A step = plan (size binning) + the batched intersection kernel over all pairs,
+inputs resident in HBM.  `roofline.achieved` = algorithmic bytes of one launch
+of the dominant kernel (4(|A|+|B|) + 20 per pair) / its mean duration, timed
+with HIP events on the kernel's own stream.  `cpu_baseline` times the CPU port
+of the reference algorithm (oracle/) on a bounded sample of the same pairs.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261203
+
+
+def _traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def _sync_all():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def bench_pairs(args):
+    import numpy as np
+    import torch
+
+    from reset_amd import api, inputs
+
+    P = args.pairs
+    t0 = time.time()
+    batch = inputs.make_pair_batch(P, seed=SEED)
+    _sync_all()
+    gen_s = time.time() - t0
+    a_off, a, b_off, b, k = batch["a_off"], batch["a"], batch["b_off"], batch["b"], batch["k"]
+    n_elems = int(a_off[-1].item()) + int(b_off[-1].item())
+    alg_bytes = 4 * n_elems + 20 * P
+
+    opts = api.default_pairs_opts()
+    plan = api.PairsPlan(P, opts)
+    sizes = torch.empty(P, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        plan.run_async(a_off, a, b_off, b, sizes)
+    _sync_all()
+    # timed region: K back-to-back steps
+    _sync_all()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run_async(a_off, a, b_off, b, sizes)
+    _sync_all()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1000.0 / args.steps
+    plan.status()
+    parity = bool(torch.equal(sizes, k))
+    # dominant-kernel duration: HIP events around the batched kernel
+    plan.set_timing(True)
+    kms = []
+    for _ in range(args.steps):
+        plan.run_async(a_off, a, b_off, b, sizes)
+        kms.append(plan.kernel_ms())
+    plan.set_timing(False)
+    parity = parity and bool(torch.equal(sizes, k))
+    kernel_ms = float(np.mean(kms))
+    achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
+
+    out = {
+        "metric": "set-pair intersections/s (1 GPU)",
+        "value": P / (ms_step / 1000.0),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded on device: log-uniform sizes, Feistel-permuted 2^24 universe)",
+        "config": {
+            "workload": "batched pairwise intersection: 1M set pairs, |A|,|B| log-uniform in [16,65536], "
+                        "sorted unique uint32 from 2^24, overlap U[0,0.5], one persistent size-binned launch",
+            "pairs": P,
+            "elements": n_elems,
+            "parallelism": "single GPU",
+        },
+        "parity": {"bit_exact": parity, "check": "sizes == constructed overlap k for all pairs"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "pairs_batched_kernel",
+            "kernel_ms": kernel_ms,
+            "alg_bytes_per_launch": alg_bytes,
+        },
+        "gen_s": gen_s,
+    }
+    tr = _traffic("pairs_batched")
+    if tr and tr.get("pairs") == P:
+        out["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
+        out["roofline"]["traffic_source"] = tr.get("source")
+
+    if not args.no_plain:
+        Np = min(args.plain_sample, P)
+        sa, sb = a_off[:Np + 1].contiguous(), b_off[:Np + 1].contiguous()
+        api.pairs_intersect_plain(sa, a, sb, b)  # warm-up
+        _sync_all()
+        t0 = time.perf_counter()
+        plain = api.pairs_intersect_plain(sa, a, sb, b)
+        _sync_all()
+        pt = time.perf_counter() - t0
+        out["plain_form"] = {
+            "value": Np / pt,
+            "unit": "pairs/s",
+            "sample_pairs": Np,
+            "bit_exact": bool(torch.equal(plain, k[:Np])),
+            "batched_speedup": (P / (ms_step / 1000.0)) / (Np / pt),
+        }
+
+    if not args.no_cpu:
+        import oracle
+
+        S = min(args.cpu_sample, P)
+        ha = a_off[:S + 1].cpu().numpy()
+        hb = b_off[:S + 1].cpu().numpy()
+        A = a[:int(ha[-1])].cpu().numpy()
+        B = b[:int(hb[-1])].cpu().numpy()
+        threads = oracle.max_threads()
+        oracle.pairs_hash(ha, A, hb, B, nthreads=threads)  # warm-up
+        t0 = time.perf_counter()
+        ref = oracle.pairs_hash(ha, A, hb, B, nthreads=threads)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": S / ct,
+            "unit": "pairs/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {S} of the {P} pairs (same data), CPU port of set_size/intersection.cu "
+                      f"(table of the larger set, smaller probes), OpenMP dynamic schedule",
+            "bit_exact_vs_gpu": bool(np.array_equal(ref, sizes[:S].cpu().numpy())),
+        }
+    plan.close()
+    return out
+
+
+def bench_tc(args):
+    import numpy as np
+    import torch
+
+    from reset_amd import api, inputs
+
+    t0 = time.time()
+    off, cols = inputs.rmat_tc_csr(args.scale, 16, seed=SEED + 3)
+    torch.cuda.empty_cache()
+    t = api.HashTables(off, cols, 0.2, 5, 3)
+    keys = t.probe_order(1)
+    probes, edges = t.tc_work()
+    _sync_all()
+    prep_s = time.time() - t0
+    out_t = torch.zeros(1, dtype=torch.int64, device="cuda")
+    res = {}
+    for name, layout, pk, plain in (("tiled_hierarchical", 1, keys, False), ("tiled_normal", 0, None, False),
+                                    ("plain_hierarchical", 1, keys, True)):
+        for _ in range(args.warmup):
+            t.tc_count_async(out_t, layout, pk, plain)
+        _sync_all()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        steps = args.steps if not plain else max(1, args.steps // 4)
+        e0.record(st)
+        for _ in range(steps):
+            t.tc_count_async(out_t, layout, pk, plain)
+        e1.record(st)
+        _sync_all()
+        t.status()
+        res[name] = (e0.elapsed_time(e1) / steps, int(out_t.item()))
+    counts = {v[1] for v in res.values()}
+    ms = res["tiled_hierarchical"][0]
+    tri = res["tiled_hierarchical"][1]
+    alg_bytes = 8 * probes + 24 * edges
+    achieved = alg_bytes / (ms / 1000.0) / 1e9
+    out = {
+        "metric": "triangles/s (1 GPU)",
+        "value": tri / (ms / 1000.0),
+        "unit": "triangles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic R-MAT (a=.57 b=.19 c=.19, seeded, labels scrambled)",
+        "config": {"workload": f"triangle counting, R-MAT scale {args.scale} edge factor 16, degree-oriented int32 CSR",
+                   "scale": args.scale, "vertices": int(off.numel() - 1), "oriented_edges": edges,
+                   "probes": probes, "parallelism": "single GPU"},
+        "triangles": tri,
+        "parity": {"layouts_agree": len(counts) == 1},
+        "pairs_per_s": edges / (ms / 1000.0),
+        "variants_ms": {k: v[0] for k, v in res.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tc_tiled_kernel",
+                     "kernel_ms": ms, "alg_bytes_per_launch": alg_bytes},
+        "prep_s": prep_s,
+    }
+    if not args.no_cpu:
+        import oracle
+
+        # bounded sample: the first rows of the oriented CSR, all their edges
+        offh = off.cpu().numpy().astype(np.int64)
+        colsh = cols.cpu().numpy()
+        threads = oracle.max_threads()
+        t0 = time.perf_counter()
+        tri_cpu = oracle.tc_merge(offh.astype(np.int32), colsh, nthreads=threads) if args.scale <= 20 else None
+        ct = time.perf_counter() - t0
+        if tri_cpu is not None:
+            out["cpu_baseline"] = {"value": tri_cpu / ct, "unit": "triangles/s", "cores": threads,
+                                   "kind": "port", "sample": "whole graph, sorted-merge CPU count",
+                                   "bit_exact_vs_gpu": tri_cpu == tri}
+    t.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "tc"])
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--cpu-sample", type=int, default=50_000)
+    ap.add_argument("--plain-sample", type=int, default=20_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args(argv)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "this build measures the single-GPU hot path only", file=sys.stderr)
+        return 2
+    import torch
+
+    from reset_amd import lib
+
+    if not torch.cuda.is_available():
+        print("bench.py needs an MI355X GPU", file=sys.stderr)
+        return 2
+    lib.load()
+    out = bench_pairs(args) if args.workload == "pairs" else bench_tc(args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -203,7 +203,9 @@ rs_status rs_ir_server_submit(rs_ir_server* s, int64_t query_id, const int32_t* 
 /* Drain up to `cap` completions; *h_n receives how many were written. */
 rs_status rs_ir_server_poll(rs_ir_server* s, rs_ir_completion* h_out, int32_t cap, int32_t* h_n);
 /* Copy a completed query's result (ascending doc ids if the index elements
- * are sorted) to h_docs (cap entries). */
+ * are sorted) to h_docs (cap entries); *h_count = its full size.  The result
+ * is released once it was copied whole; call again with a larger buffer when
+ * *h_count > cap. */
 rs_status rs_ir_server_result(rs_ir_server* s, int64_t query_id, int32_t* h_docs, int64_t cap,
                               int64_t* h_count);
 /* Block until every submitted query completed. */
@@ -262,6 +264,12 @@ rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, co
                                  const int64_t* d_b_off, const int32_t* d_b, int64_t num_pairs,
                                  int32_t* d_sizes, rs_stream stream);
 rs_status rs_pairs_graph_launch(rs_pairs_plan* plan, rs_stream stream);
+/* Kernel timing: when enabled, rs_pairs_intersect_batched_async records HIP
+ * events on its stream around the batched kernel (not the plan kernels);
+ * rs_pairs_plan_kernel_ms waits for and returns the last launch's duration.
+ * Must be disabled while capturing a graph. */
+rs_status rs_pairs_plan_timing(rs_pairs_plan* p, int32_t enable);
+rs_status rs_pairs_plan_kernel_ms(rs_pairs_plan* p, float* h_ms);
 /* Device status of the plan's last launches (RS_EOVERFLOW on a full table). */
 rs_status rs_pairs_plan_status(rs_pairs_plan* p, rs_stream stream);
 

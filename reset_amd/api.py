@@ -242,6 +242,15 @@ class PairsPlan:
     def status(self, stream=None):
         check(load().rs_pairs_plan_status(self._h, stream_handle(stream)))
 
+    def set_timing(self, enable=True):
+        check(load().rs_pairs_plan_timing(self._h, int(bool(enable))))
+
+    def kernel_ms(self):
+        """Duration of the last batched kernel (HIP events on its stream)."""
+        ms = C.c_float(0)
+        check(load().rs_pairs_plan_kernel_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
     def close(self):
         if self._h:
             load().rs_pairs_plan_free(self._h)

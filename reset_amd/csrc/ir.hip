@@ -548,7 +548,7 @@ rs_status rs_ir_server_result(rs_ir_server* S, int64_t query_id, int32_t* h_docs
   RS_REQUIRE(h_docs || it->second.empty() || cap == 0, "rs_ir_server_result: null buffer");
   const int64_t n = *h_count < cap ? *h_count : cap;
   if (n > 0) memcpy(h_docs, it->second.data(), sizeof(int32_t) * (size_t)n);
-  S->results.erase(it);
+  if (n == *h_count) S->results.erase(it);  // kept until fully copied out
   return RS_OK;
 }
 

@@ -283,9 +283,13 @@ struct rs_pairs_plan {
   DevBuf ctl;     // int32[kCtlWords]
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the batched kernel
   ~rs_pairs_plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
   }
 };
 
@@ -324,8 +328,10 @@ static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, cons
     plan_bins_scan<<<1, 64, 0, st>>>(nb, ctl);
     plan_scatter<<<g, kPlanBlock, 0, st>>>(d_a_off, d_b_off, P, nb, ctl, order);
     const int blocks = device_cu_count() * plan->opts.waves_per_cu / kWavesPerBlock;
+    if (plan->timing) RS_HIP(hipEventRecord(plan->ev0, st));
     pairs_batched_kernel<<<blocks, kPairBlock, 0, st>>>(d_a_off, d_a, d_b_off, d_b, P, order, ctl,
                                                         d_sizes, plan->cfg);
+    if (plan->timing) RS_HIP(hipEventRecord(plan->ev1, st));
   }
   RS_HIP(hipGetLastError());
   return RS_OK;
@@ -420,6 +426,7 @@ rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, co
     plan->graph = nullptr;
   }
   (void)device_cu_count();  // attribute query outside the capture
+  RS_REQUIRE(!plan->timing, "rs_pairs_graph_capture: disable kernel timing before capturing");
   RS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   rs_status s = pairs_enqueue(plan, d_a_off, d_a, d_b_off, d_b, num_pairs, d_sizes, st);
   hipGraph_t g = nullptr;
@@ -440,6 +447,23 @@ rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, co
 rs_status rs_pairs_graph_launch(rs_pairs_plan* plan, rs_stream stream) {
   RS_REQUIRE(plan && plan->exec, "rs_pairs_graph_launch: no captured graph");
   RS_HIP(hipGraphLaunch(plan->exec, as_stream(stream)));
+  return RS_OK;
+}
+
+rs_status rs_pairs_plan_timing(rs_pairs_plan* p, int32_t enable) {
+  RS_REQUIRE(p, "rs_pairs_plan_timing: null plan");
+  if (enable && !p->ev0) {
+    RS_HIP(hipEventCreate(&p->ev0));
+    RS_HIP(hipEventCreate(&p->ev1));
+  }
+  p->timing = enable != 0;
+  return RS_OK;
+}
+
+rs_status rs_pairs_plan_kernel_ms(rs_pairs_plan* p, float* h_ms) {
+  RS_REQUIRE(p && h_ms && p->ev0, "rs_pairs_plan_kernel_ms: timing was never enabled");
+  RS_HIP(hipEventSynchronize(p->ev1));
+  RS_HIP(hipEventElapsedTime(h_ms, p->ev0, p->ev1));
   return RS_OK;
 }
 

@@ -1,0 +1,77 @@
+"""Python handle of the native IR serving loop (rs_ir_server_* in the C ABI).
+
+The loop itself -- admission by memory footprint, batching, overlapping
+launches on per-slot streams, completion -- runs in the library's own host
+thread; this class only submits queries and reads completions/results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _L
+from .lib import NORMAL, check, load
+
+
+class IRServer:
+    def __init__(self, tables, layout=NORMAL, budget_bytes=1 << 30, max_batch=256, max_inflight=4):
+        self.tables = tables  # keeps the index and its tables alive
+        self._h = C.c_void_p()
+        check(load().rs_ir_server_create(tables.handle, int(layout), int(budget_bytes), int(max_batch),
+                                         int(max_inflight), C.byref(self._h)))
+        self._buf = (_L.rs_ir_completion * 4096)()
+
+    def submit(self, query_id, terms):
+        arr = np.ascontiguousarray(terms, dtype=np.int32)
+        check(load().rs_ir_server_submit(self._h, int(query_id), arr.ctypes.data_as(C.c_void_p), len(arr)))
+
+    def poll(self):
+        n = C.c_int32(0)
+        check(load().rs_ir_server_poll(self._h, self._buf, len(self._buf), C.byref(n)))
+        return [self._buf[i] for i in range(n.value)]
+
+    def poll_all(self):
+        out = []
+        while True:
+            got = self.poll()
+            if not got:
+                return out
+            out.extend(_copy(c) for c in got)
+
+    def result(self, query_id):
+        """The query's doc ids (releases the stored result)."""
+        cap = 1 << 14
+        while True:
+            buf = np.empty(cap, np.int32)
+            cnt = C.c_int64(0)
+            check(load().rs_ir_server_result(self._h, int(query_id), buf.ctypes.data_as(C.c_void_p), cap,
+                                             C.byref(cnt)))
+            if cnt.value <= cap:
+                return buf[:cnt.value].tolist()
+            cap = int(cnt.value)
+
+    def drain(self):
+        check(load().rs_ir_server_drain(self._h))
+
+    def stats(self):
+        a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        check(load().rs_ir_server_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"launches": a.value, "queries": b.value, "peak_bytes": c.value}
+
+    def close(self):
+        if self._h:
+            load().rs_ir_server_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _copy(c):
+    out = _L.rs_ir_completion()
+    C.pointer(out)[0] = c
+    return out
