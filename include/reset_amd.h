@@ -220,16 +220,18 @@ void rs_ir_server_free(rs_ir_server* s);
  * families with int64 offsets.  Sets need not be sorted for the hash probe;
  * the batched form's range chunking requires ascending sets (flag below). */
 typedef struct rs_pairs_bin {
-  float load_factor;   /* alpha of the per-pair table                      */
-  int32_t bucket_size; /* b                                                */
-  int32_t chunk;       /* max elements of the build side per LDS table     */
-  int32_t unroll;      /* probe-side elements per lane per pass (1,2,4)    */
+  float load_factor;   /* alpha of the per-pair LDS table                  */
+  int32_t bucket_size; /* b: slots per bucket, 1, 2 or 4 (one LDS read)    */
+  int32_t chunk;       /* max build-side elements per table, <= 1024       */
+  int32_t unroll;      /* 16-byte loads per lane in flight (1, 2, 4); the  */
+                       /* same in every bin                                */
 } rs_pairs_bin;
 #define RS_PAIRS_MAX_BINS 20
 typedef struct rs_pairs_opts {
   int32_t num_bins;                 /* bins by floor(log2(|A|+|B|)), <= 20   */
   rs_pairs_bin bins[RS_PAIRS_MAX_BINS];
-  int32_t waves_per_cu;             /* persistent grid occupancy            */
+  int32_t waves_per_cu;             /* persistent grid occupancy (4..32)    */
+  int32_t lds_slots_per_wave;       /* int32 LDS slots of a wave's table    */
 } rs_pairs_opts;
 /* Defaults tuned for gfx950 (also the autotuner's starting point). */
 void rs_pairs_default_opts(rs_pairs_opts* o);

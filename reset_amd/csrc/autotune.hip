@@ -30,19 +30,18 @@ struct Cand {
 
 std::vector<Cand> candidates() {
   const float alphas[] = {0.2f, 0.25f, 0.5f, 0.75f, 0.9f};
-  const int bs[] = {1, 2, 4, 5};
-  const int chunks[] = {255, 511, 1023};
-  const int unrolls[] = {4, 8};
+  const int bs[] = {4};
+  const int chunks[] = {255, 511, 1024};
   std::vector<Cand> out;
   for (float a : alphas)
     for (int b : bs)
       for (int c : chunks) {
         const int32_t L = calc_length(c, a, b);
-        if (L <= 0 || (int64_t)L * b > 2048) continue;
+        if (L <= 0 || (int64_t)L * b > 2048) continue;  // default LDS region
         // fraction of slots in use when the chunk is full
         const double load = (double)c / ((double)L * b);
         const double est = 1.0 + load * b / (1.0 + (b - 1) * (1.0 - load)) + 64.0 / c;
-        for (int u : unrolls) out.push_back(Cand{rs_pairs_bin{a, b, c, u}, est});
+        out.push_back(Cand{rs_pairs_bin{a, b, c, 0}, est});
       }
   std::stable_sort(out.begin(), out.end(), [](const Cand& x, const Cand& y) { return x.est < y.est; });
   if (out.size() > 16) out.resize(16);
@@ -88,9 +87,13 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
     }
     float best = 1e30f;
     rs_pairs_bin best_bin = opts->bins[bin];
+    best_bin.unroll = opts->bins[0].unroll;
     for (const Cand& c : cands) {
       rs_pairs_opts trial = *opts;
-      for (int k = 0; k < nb; ++k) trial.bins[k] = c.bin;
+      for (int k = 0; k < nb; ++k) {
+        trial.bins[k] = c.bin;
+        trial.bins[k].unroll = opts->bins[0].unroll;
+      }
       float ms_min = 1e30f;
       for (int r = 0; r <= reps && result == RS_OK; ++r) {  // r == 0 warms up
         (void)hipEventRecord(e0, st);

@@ -21,7 +21,6 @@ namespace rs {
 
 constexpr int kPairBlock = 256;
 constexpr int kWavesPerBlock = kPairBlock / kWave;
-constexpr int kLdsSlotsPerWave = 2048;  // 8 KiB per wave -> 20 waves per CU
 constexpr int kPlanBlock = 256;
 
 struct PairBinDev {
@@ -80,152 +79,369 @@ __global__ void plan_bins_scan(int num_bins, int32_t* ctl) {
   }
 }
 
-__global__ void plan_scatter(const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
-                             int64_t P, int num_bins, int32_t* ctl, int32_t* __restrict__ order) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t tot = (a_off[i + 1] - a_off[i]) + (b_off[i + 1] - b_off[i]);
-    const int32_t pos = atomicAdd(&ctl[kCtlCursor + pair_bin(tot, num_bins)], 1);
-    order[pos] = (int32_t)i;
+// Block-aggregated scatter: ranks inside the block come from LDS atomics,
+// then one global atomic per (block, bin) reserves the block's range.
+__global__ void __launch_bounds__(kPlanBlock) plan_scatter(const int64_t* __restrict__ a_off,
+                                                           const int64_t* __restrict__ b_off,
+                                                           int64_t P, int num_bins, int32_t* ctl,
+                                                           int32_t* __restrict__ order) {
+  __shared__ int32_t cnt[RS_PAIRS_MAX_BINS];
+  __shared__ int32_t base[RS_PAIRS_MAX_BINS];
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < P; i0 += (int64_t)gridDim.x * blockDim.x) {
+    if (threadIdx.x < RS_PAIRS_MAX_BINS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    int bin = 0, rank = 0;
+    if (i < P) {
+      const int64_t tot = (a_off[i + 1] - a_off[i]) + (b_off[i + 1] - b_off[i]);
+      bin = pair_bin(tot, num_bins);
+      rank = atomicAdd(&cnt[bin], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < num_bins && cnt[threadIdx.x] > 0)
+      base[threadIdx.x] = atomicAdd(&ctl[kCtlCursor + threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (i < P) order[base[bin] + rank] = (int32_t)i;
+    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------- LDS table
-__device__ __forceinline__ void lds_clear(int32_t* tab, int32_t slots, int lane) {
-  for (int32_t s = lane; s < slots; s += kWave) tab[s] = kEmpty;
-}
+// ---------------------------------------------------------------- LDS tables
+// A wave's transient table lives in its private LDS region, bucket-major:
+// slot k of bucket j at tab[j*B + k], so one B-slot bucket is one 4/8/16-byte
+// LDS read.  B (slots per bucket) is the reference's bucket_size and the
+// bucket count L follows its length rule (hashtable.cu:20-36); home bucket
+// x & (L-1), then the following buckets (hashtable.cu:53-101, TC.cu:11-31).
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
 
-// hashtable.cu:53-101 inside one wave's LDS region, BN = L.
-__device__ __forceinline__ bool lds_insert(int32_t* tab, uint32_t L, int32_t b, int32_t v) {
-  uint32_t bucket = v & (L - 1u);
-  int32_t k = 0;
-  const uint32_t bound = L * (uint32_t)b;
-  for (uint32_t tries = 0; tries < bound; ++tries) {
-    if (atomicCAS(&tab[bucket + (uint32_t)k * L], kEmpty, v) == kEmpty) return true;
-    if (++k == b) {
-      k = 0;
-      bucket = (bucket + 1u) & (L - 1u);
-    }
-  }
-  return false;
-}
-
-// TC.cu:11-31 inside LDS.  1 found, 0 absent, -1 bound exhausted.
-__device__ __forceinline__ int lds_probe(const int32_t* tab, uint32_t L, int32_t b, int32_t key) {
-  uint32_t bucket = key & (L - 1u);
-  int32_t k = 0;
-  const uint32_t bound = L * (uint32_t)b;
-  for (uint32_t s = 0; s < bound; ++s) {
-    const int32_t v = tab[bucket + (uint32_t)k * L];
-    if (v == key) return 1;
-    if (v == kEmpty) return 0;
-    if (++k == b) {
-      k = 0;
-      bucket = (bucket + 1u) & (L - 1u);
-    }
+// Insert; returns 0 if v landed in its home bucket, 1 if it was displaced to
+// a later bucket (the home bucket was full), -1 if the table had no free slot.
+template <int B>
+__device__ __forceinline__ int lds_insert(int32_t* tab, uint32_t L, int32_t v) {
+  uint32_t bucket = (uint32_t)v & (L - 1u);
+  for (uint32_t step = 0; step < L; ++step) {
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+      if (atomicCAS(&tab[bucket * B + k], kEmpty, v) == kEmpty) return step == 0 ? 0 : 1;
+    bucket = (bucket + 1u) & (L - 1u);
   }
   return -1;
 }
 
-// One pair inside one wave.  Returns this lane's hit count.
-template <int UNROLL>
-__device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restrict__ S, int64_t ns,
-                                                        const int32_t* __restrict__ Gp, int64_t ng,
-                                                        int32_t* tab, const PairBinDev bin,
-                                                        bool& bad) {
-  const int lane = __lane_id();
+template <int B>
+__device__ __forceinline__ void lds_read_bucket(const int32_t* tab, uint32_t bucket, int32_t (&o)[B]) {
+  if constexpr (B == 4) {
+    const v4i v = *reinterpret_cast<const v4i*>(tab + bucket * 4);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+  } else if constexpr (B == 2) {
+    const v2i v = *reinterpret_cast<const v2i*>(tab + bucket * 2);
+    o[0] = v.x; o[1] = v.y;
+  } else {
+    o[0] = tab[bucket];
+  }
+}
+
+// N keys of one lane looked up together: every step reads one bucket per
+// pending key (N LDS reads in flight), resolving a key on a match or on an
+// EMPTY slot in its bucket; a full bucket without the key moves it on.
+template <int B, int N>
+__device__ __forceinline__ uint32_t lds_lookup_n(const int32_t* tab, uint32_t L,
+                                                 const int32_t (&x)[N], const bool (&act)[N],
+                                                 bool& bad) {
+  const uint32_t mask = L - 1u;
+  uint32_t bucket[N];
+  bool pend[N];
   uint32_t hits = 0;
-  int64_t gpos = 0;
-  for (int64_t c0 = 0; c0 < ns; c0 += bin.chunk) {
-    const int64_t cl = ns - c0 < bin.chunk ? ns - c0 : bin.chunk;
-    const bool last = c0 + cl >= ns;
-    const int64_t hi = last ? (int64_t)1 << 40 : (int64_t)S[c0 + cl];
-    const uint32_t L = (uint32_t)calc_length((int32_t)cl, bin.alpha, bin.b);
-    lds_clear(tab, (int32_t)(L * (uint32_t)bin.b), lane);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    for (int64_t j = lane; j < cl; j += kWave) bad |= !lds_insert(tab, L, bin.b, S[c0 + j]);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // stream the larger set from gpos while its elements are < hi
-    while (gpos < ng) {
-      int32_t x[UNROLL];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) {
-        const int64_t j = gpos + u * kWave + lane;
-        x[u] = j < ng ? Gp[j] : 0x7fffffff;
-      }
-      int64_t stop = -1;
+  for (int n = 0; n < N; ++n) {
+    bucket[n] = (uint32_t)x[n] & mask;
+    pend[n] = act[n];
+  }
+  for (uint32_t step = 0; step < L; ++step) {
+    int32_t v[N][B];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) {
-        const int64_t j = gpos + u * kWave + lane;
-        const bool in = j < ng && (int64_t)x[u] < hi;
-        if (in) {
-          const int r = lds_probe(tab, L, bin.b, x[u]);
-          hits += r == 1;
-          bad |= r < 0;
-        }
-        const uint64_t beyond = __ballot(j < ng && !((int64_t)x[u] < hi));
-        if (stop < 0 && beyond) stop = gpos + u * kWave + (__ffsll((unsigned long long)beyond) - 1);
+    for (int n = 0; n < N; ++n) lds_read_bucket<B>(tab, bucket[n], v[n]);
+    bool left = false;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      bool found = false, empty = false;
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        found |= v[n][k] == x[n];
+        empty |= v[n][k] == kEmpty;
       }
-      if (stop >= 0) {
-        gpos = stop;
-        break;
+      if (pend[n]) {
+        hits += found;
+        pend[n] = !(found || empty);
+        left |= pend[n];
+        bucket[n] = (bucket[n] + 1u) & mask;
       }
-      gpos += (int64_t)UNROLL * kWave;
     }
-    __builtin_amdgcn_wave_barrier();
+    if (!__any(left)) return hits;
+  }
+  bad = true;
+  return hits;
+}
+
+// Lookups when no key of the table was displaced: a key present in the table
+// is in its home bucket, so one bucket read decides it (no EMPTY test, no
+// walk).  any(slot == x) is evaluated as min(slot ^ x) == 0 in vector ALU.
+template <int B, int N>
+__device__ __forceinline__ uint32_t lds_lookup_home(const int32_t* tab, uint32_t L,
+                                                    const int32_t (&x)[N], const bool (&act)[N]) {
+  const uint32_t mask = L - 1u;
+  int32_t v[N][B];
+#pragma unroll
+  for (int n = 0; n < N; ++n) lds_read_bucket<B>(tab, (uint32_t)x[n] & mask, v[n]);
+  uint32_t hits = 0;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
+#pragma unroll
+    for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
+    hits += (uint32_t)(m == 0u) & (uint32_t)act[n];
   }
   return hits;
 }
 
-template <int UNROLL>
-__device__ __forceinline__ uint32_t pair_dispatch(const int32_t* S, int64_t ns, const int32_t* Gp,
-                                                  int64_t ng, int32_t* tab, const PairBinDev& bin,
-                                                  bool& bad) {
-  return pair_intersect_wave<UNROLL>(S, ns, Gp, ng, tab, bin, bad);
+// First index j in [lo, hi) with (uint32)G[j] >= key (G ascending), found
+// with 64-ary sampling: every round the 64 lanes test evenly spaced positions
+// and the ballot (a prefix of lanes) narrows the range ~65x.
+__device__ __forceinline__ int32_t wave_lower_bound(const int32_t* __restrict__ G, int32_t lo,
+                                                    int32_t hi, uint32_t key) {
+  const int lane = __lane_id();
+  while (hi - lo > kWave) {
+    const int64_t span = hi - lo;
+    const int32_t p = lo + (int32_t)((span * (lane + 1)) / (kWave + 1));
+    const uint64_t m = __ballot((uint32_t)G[p] < key);
+    const int c = __popcll(m);
+    const int32_t plo = __shfl(p, c > 0 ? c - 1 : 0, kWave);
+    const int32_t phi = __shfl(p, c < kWave ? c : kWave - 1, kWave);
+    if (c > 0) lo = plo + 1;
+    if (c < kWave) hi = phi;
+  }
+  const bool less = lo + lane < hi && (uint32_t)G[lo + lane] < key;
+  return lo + __popcll(__ballot(less));
 }
 
+constexpr int kMaxChunkPerLane = 16;  // chunk <= 1024 build-side elements
+
+// One block of the larger set: 4*U elements per lane, 256*U per wave, from a
+// 16-byte aligned position `pos` (which may lie up to 3 elements before the
+// set's first element: those lanes are inactive).  Element e is valid if
+// first <= e < n.
+template <int U>
+struct GBlock {
+  int32_t x[4 * U];
+  bool act[4 * U];
+};
+
+template <int U>
+__device__ __forceinline__ void load_block(const int32_t* __restrict__ G, int32_t pos, int32_t first,
+                                           int32_t n, GBlock<U>& blk) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int32_t e = pos + 4 * (u * kWave + lane);  // first element of this v4
+    v4i v = {kEmpty, kEmpty, kEmpty, kEmpty};
+    if (e + 3 < n && e >= first) {
+      v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(G + e));
+    } else {
+      if (e >= first && e < n) v.x = G[e];
+      if (e + 1 >= first && e + 1 < n) v.y = G[e + 1];
+      if (e + 2 >= first && e + 2 < n) v.z = G[e + 2];
+      if (e + 3 >= first && e + 3 < n) v.w = G[e + 3];
+    }
+    blk.x[4 * u] = v.x;
+    blk.x[4 * u + 1] = v.y;
+    blk.x[4 * u + 2] = v.z;
+    blk.x[4 * u + 3] = v.w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) blk.act[4 * u + c] = e + c >= first && e + c < n;
+  }
+}
+
+// Fills the wave's LDS table with chunk [c0, c0+cl) of the build side, held in
+// registers (sv[r] = S[c0 + r*64 + lane]).  Returns whether some key had to
+// leave its home bucket.
+template <int B, int CPL>
+__device__ __forceinline__ bool build_chunk(int32_t* tab, uint32_t L, const int32_t (&sv)[CPL],
+                                            int32_t cl, bool& bad) {
+  const int lane = __lane_id();
+  const int32_t slots = (int32_t)L * B;
+  for (int32_t s4 = lane; s4 < (slots >> 2); s4 += kWave)
+    *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
+  __builtin_amdgcn_wave_barrier();
+  bool moved = false;
+#pragma unroll
+  for (int r = 0; r < CPL; ++r)
+    if (r * kWave + lane < cl) {
+      const int rc = lds_insert<B>(tab, L, sv[r]);
+      bad |= rc < 0;
+      moved |= rc != 0;
+    }
+  __builtin_amdgcn_wave_barrier();
+  return __any(moved);
+}
+
+template <int CPL>
+__device__ __forceinline__ void load_chunk(const int32_t* __restrict__ S, int32_t c0, int32_t cl,
+                                           int32_t (&sv)[CPL]) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int r = 0; r < CPL; ++r) {
+    const int32_t j = r * kWave + lane;
+    sv[r] = j < cl ? S[c0 + j] : kEmpty;
+  }
+}
+
+template <int B, int U>
+__device__ __forceinline__ uint32_t probe_block(const int32_t* tab, uint32_t L, bool displaced,
+                                                const int32_t (&x)[4 * U], const bool (&in)[4 * U],
+                                                bool& bad) {
+  if (!displaced) return lds_lookup_home<B, 4 * U>(tab, L, x, in);
+  // rare: some key of this chunk left its home bucket; walk key by key
+  uint32_t hits = 0;
+  for (int i = 0; i < 4 * U; ++i) {
+    int32_t x1[1] = {x[i]};
+    bool a1[1] = {in[i]};
+    hits += lds_lookup_n<B, 1>(tab, L, x1, a1, bad);
+  }
+  return hits;
+}
+
+// One pair inside one wave.  The larger set G is streamed exactly once as a
+// sequence of aligned blocks (the next block is loaded while the current one
+// is probed).  The smaller set S is built into the LDS table chunk by chunk;
+// chunk c covers the values [S[c0], S[c0+cl]), so each block element is
+// probed against the chunk whose range holds it.  When a block crosses into
+// the next chunk, the table is rebuilt from the next chunk's elements, which
+// were prefetched into registers, and the same block's remaining elements are
+// probed again -- chunk switches cost LDS work only, never a global round trip.
+template <int B, int U, int CPL>
+__device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restrict__ S, int32_t ns,
+                                                        const int32_t* __restrict__ G, int32_t ng,
+                                                        int32_t* tab, float alpha, int32_t chunk,
+                                                        bool& bad) {
+  constexpr int32_t kBlock = 4 * U * kWave;
+  uint32_t hits = 0;
+  // current chunk [c0, c0+cl) with upper bound hi; the next chunk [n0, n0+nl)
+  // is prefetched into sn together with its own upper bound nhi
+  int32_t c0 = 0;
+  int32_t cl = ns < chunk ? ns : chunk;
+  int32_t sv[CPL], sn[CPL];
+  load_chunk<CPL>(S, 0, cl, sv);
+  bool last = cl >= ns;
+  uint32_t hi = last ? 0xFFFFFFFFu : (uint32_t)S[cl];
+  int32_t n0 = cl, nl = 0;
+  uint32_t nhi = 0xFFFFFFFFu;
+  if (!last) {
+    nl = ns - n0 < chunk ? ns - n0 : chunk;
+    load_chunk<CPL>(S, n0, nl, sn);
+    if (n0 + nl < ns) nhi = (uint32_t)S[n0 + nl];
+  }
+  // aligned start of G (at most 3 elements before G[0], same allocation)
+  const int32_t mis = (int32_t)((reinterpret_cast<uintptr_t>(G) >> 2) & 3u);
+  int32_t pos = -mis;
+  GBlock<U> cur;
+  load_block<U>(G, pos, 0, ng, cur);
+  uint32_t L = (uint32_t)calc_length(cl, alpha, B);
+  bool displaced = build_chunk<B, CPL>(tab, L, sv, cl, bad);
+  while (true) {
+    GBlock<U> nxt;
+    const bool more = pos + kBlock < ng;
+    if (more) load_block<U>(G, pos + kBlock, 0, ng, nxt);
+    while (true) {
+      bool in[4 * U];
+      bool rest = false;
+#pragma unroll
+      for (int i = 0; i < 4 * U; ++i) {
+        const bool below = (uint32_t)cur.x[i] < hi;
+        in[i] = cur.act[i] && below;
+        cur.act[i] = cur.act[i] && !below;
+        rest |= cur.act[i];
+      }
+      hits += probe_block<B, U>(tab, L, displaced, cur.x, in, bad);
+      if (last || !__any(rest)) break;
+      // the block reaches into the next chunk: switch tables, probe again
+      c0 = n0;
+      cl = nl;
+      hi = nhi;
+      last = c0 + cl >= ns;
+#pragma unroll
+      for (int r = 0; r < CPL; ++r) sv[r] = sn[r];
+      if (!last) {
+        n0 = c0 + cl;
+        nl = ns - n0 < chunk ? ns - n0 : chunk;
+        load_chunk<CPL>(S, n0, nl, sn);
+        nhi = n0 + nl < ns ? (uint32_t)S[n0 + nl] : 0xFFFFFFFFu;
+      }
+      L = (uint32_t)calc_length(cl, alpha, B);
+      __builtin_amdgcn_wave_barrier();
+      displaced = build_chunk<B, CPL>(tab, L, sv, cl, bad);
+    }
+    if (!more) break;
+    pos += kBlock;
+    cur = nxt;
+  }
+  __builtin_amdgcn_wave_barrier();
+  return hits;
+}
+
+// Persistent kernel: wave w takes the pairs at positions w, w + W, w + 2W, ...
+// of the size-sorted order (largest first), so every wave gets a near-equal
+// mix and no global atomic is needed; the next pair's offsets are fetched
+// before the current pair is processed.
+template <int B, int U, int CPL>
 __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     const int64_t* __restrict__ a_off, const int32_t* __restrict__ a,
     const int64_t* __restrict__ b_off, const int32_t* __restrict__ b, int64_t P,
     const int32_t* __restrict__ order, int32_t* ctl, int32_t* __restrict__ sizes,
-    const PairsConst cfg) {
-  __shared__ int32_t lds[kWavesPerBlock * kLdsSlotsPerWave];
+    const PairsConst cfg, int32_t slots_per_wave) {
+  extern __shared__ __attribute__((aligned(16))) int32_t lds_dyn[];
   const int lane = __lane_id();
-  const int wid = threadIdx.x / kWave;
-  int32_t* tab = lds + wid * kLdsSlotsPerWave;
+  // wave-uniform: pair index, offsets and loop bounds stay in SGPRs
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  int32_t* tab = lds_dyn + wid * slots_per_wave;
+  const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
+  int64_t idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
   bool bad = false;
-  while (true) {
-    int32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(&ctl[kCtlCounter], 1);
-    idx = __shfl(idx, 0, kWave);
-    if (idx >= P) break;
-    const int32_t i = order[idx];
-    const int64_t a0 = a_off[i], a1 = a_off[i + 1], b0 = b_off[i], b1 = b_off[i + 1];
-    int64_t ns = a1 - a0, ng = b1 - b0;
+  int32_t i = 0;
+  int64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+  if (idx < P) {
+    i = order[idx];
+    a0 = a_off[i]; a1 = a_off[i + 1]; b0 = b_off[i]; b1 = b_off[i + 1];
+  }
+  for (; idx < P; idx += W) {
+    // prefetch the next pair's metadata
+    const int64_t nidx = idx + W;
+    int32_t ni = 0;
+    int64_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+    if (nidx < P) {
+      ni = order[nidx];
+      na0 = a_off[ni]; na1 = a_off[ni + 1]; nb0 = b_off[ni]; nb1 = b_off[ni + 1];
+    }
+    int32_t ns = (int32_t)(a1 - a0), ng = (int32_t)(b1 - b0);
     const int32_t* S = a + a0;
     const int32_t* Gp = b + b0;
     if (ns > ng) {
       const int32_t* t = S;
       S = Gp;
       Gp = t;
-      const int64_t tn = ns;
+      const int32_t tn = ns;
       ns = ng;
       ng = tn;
     }
     uint32_t hits = 0;
     if (ns > 0) {
-      const PairBinDev bin = cfg.bins[pair_bin(ns + ng, cfg.num_bins)];
-      switch (bin.unroll) {
-        case 1: hits = pair_dispatch<1>(S, ns, Gp, ng, tab, bin, bad); break;
-        case 2: hits = pair_dispatch<2>(S, ns, Gp, ng, tab, bin, bad); break;
-        case 4: hits = pair_dispatch<4>(S, ns, Gp, ng, tab, bin, bad); break;
-        default: hits = pair_dispatch<8>(S, ns, Gp, ng, tab, bin, bad); break;
-      }
+      const PairBinDev bin = cfg.bins[pair_bin((int64_t)ns + ng, cfg.num_bins)];
+      hits = pair_intersect_wave<B, U, CPL>(S, ns, Gp, ng, tab, bin.alpha, bin.chunk, bad);
     }
     const uint32_t total = wave_sum(hits);
     if (lane == 0) sizes[i] = (int32_t)total;
+    i = ni; a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
   }
   if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
 }
@@ -296,23 +512,71 @@ struct rs_pairs_plan {
 static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   RS_REQUIRE(o->num_bins >= 1 && o->num_bins <= RS_PAIRS_MAX_BINS, "pairs: bad num_bins %d",
              o->num_bins);
-  RS_REQUIRE(o->waves_per_cu >= kWavesPerBlock && o->waves_per_cu <= 32,
-             "pairs: waves_per_cu must be in [4, 32]");
+  RS_REQUIRE(o->waves_per_cu >= kWavesPerBlock && o->waves_per_cu <= 32 &&
+                 o->waves_per_cu % kWavesPerBlock == 0,
+             "pairs: waves_per_cu must be a multiple of 4 in [4, 32]");
+  RS_REQUIRE(o->lds_slots_per_wave >= 256 && o->lds_slots_per_wave <= 8192 &&
+                 (o->lds_slots_per_wave & 3) == 0,
+             "pairs: lds_slots_per_wave must be a multiple of 4 in [256, 8192]");
+  RS_REQUIRE((int64_t)o->lds_slots_per_wave * 4 * o->waves_per_cu <= 160 * 1024,
+             "pairs: %d waves x %d slots exceed the 160 KiB LDS of a CU", o->waves_per_cu,
+             o->lds_slots_per_wave);
   cfg->num_bins = o->num_bins;
   for (int i = 0; i < o->num_bins; ++i) {
     const rs_pairs_bin& bn = o->bins[i];
-    RS_REQUIRE(bn.load_factor > 0.f && bn.bucket_size >= 1 && bn.bucket_size <= 16 &&
-                   bn.chunk >= 1,
-               "pairs: bad parameters in bin %d", i);
-    RS_REQUIRE(bn.unroll == 1 || bn.unroll == 2 || bn.unroll == 4 || bn.unroll == 8,
-               "pairs: unroll must be 1, 2, 4 or 8 (bin %d)", i);
+    RS_REQUIRE(bn.load_factor > 0.f && (bn.bucket_size == 1 || bn.bucket_size == 2 ||
+                                        bn.bucket_size == 4) && bn.chunk >= 1 &&
+                   bn.chunk <= kMaxChunkPerLane * kWave,
+               "pairs: bin %d needs bucket_size in {1,2,4} and chunk in [1, %d]", i,
+               kMaxChunkPerLane * kWave);
+    RS_REQUIRE(bn.unroll == o->bins[0].unroll && (bn.unroll == 1 || bn.unroll == 2 || bn.unroll == 4),
+               "pairs: unroll must be 1, 2 or 4 and the same in every bin (bin %d)", i);
+    RS_REQUIRE(bn.bucket_size == o->bins[0].bucket_size,
+               "pairs: bucket_size must be the same in every bin (bin %d)", i);
     const int32_t L = calc_length(bn.chunk, bn.load_factor, bn.bucket_size);
-    RS_REQUIRE(L > 0 && (int64_t)L * bn.bucket_size <= kLdsSlotsPerWave,
+    RS_REQUIRE(L > 0 && (int64_t)L * bn.bucket_size <= o->lds_slots_per_wave,
                "pairs: bin %d table (%d buckets x %d slots) exceeds the %d-slot LDS region", i, L,
-               bn.bucket_size, kLdsSlotsPerWave);
+               bn.bucket_size, o->lds_slots_per_wave);
     cfg->bins[i] = PairBinDev{bn.load_factor, bn.bucket_size, bn.chunk, bn.unroll};
   }
   return RS_OK;
+}
+
+template <int B, int U>
+static void launch_bu(int cpl, int blocks, size_t lds, const int64_t* a_off, const int32_t* a,
+                      const int64_t* b_off, const int32_t* b, int64_t P, const int32_t* order,
+                      int32_t* ctl, int32_t* sizes, const PairsConst& cfg, int32_t spw,
+                      hipStream_t st) {
+  if (cpl <= 4)
+    pairs_batched_kernel<B, U, 4><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+  else if (cpl <= 8)
+    pairs_batched_kernel<B, U, 8><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+  else
+    pairs_batched_kernel<B, U, 16><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+}
+
+// Launch-level shape: bucket size B and unroll U are the same in every bin;
+// CPL (build-side registers per lane) covers the largest chunk of any bin.
+static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const int64_t* a_off,
+                           const int32_t* a, const int64_t* b_off, const int32_t* b, int64_t P,
+                           const int32_t* order, int32_t* ctl, int32_t* sizes, hipStream_t st) {
+  const int blocks = device_cu_count() * o.waves_per_cu / kWavesPerBlock;
+  int max_chunk = 1;
+  for (int i = 0; i < o.num_bins; ++i) max_chunk = o.bins[i].chunk > max_chunk ? o.bins[i].chunk : max_chunk;
+  const int cpl = (max_chunk + kWave - 1) / kWave;
+  const int32_t spw = o.lds_slots_per_wave;
+  const size_t lds = sizeof(int32_t) * (size_t)spw * kWavesPerBlock;
+  const int B = o.bins[0].bucket_size, U = o.bins[0].unroll;
+#define RS_LAUNCH_BU(b_, u_) \
+  launch_bu<b_, u_>(cpl, blocks, lds, a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw, st)
+  if (B == 4) {
+    if (U == 1) RS_LAUNCH_BU(4, 1); else if (U == 2) RS_LAUNCH_BU(4, 2); else RS_LAUNCH_BU(4, 4);
+  } else if (B == 2) {
+    if (U == 1) RS_LAUNCH_BU(2, 1); else if (U == 2) RS_LAUNCH_BU(2, 2); else RS_LAUNCH_BU(2, 4);
+  } else {
+    if (U == 1) RS_LAUNCH_BU(1, 1); else if (U == 2) RS_LAUNCH_BU(1, 2); else RS_LAUNCH_BU(1, 4);
+  }
+#undef RS_LAUNCH_BU
 }
 
 static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,
@@ -327,10 +591,8 @@ static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, cons
     plan_hist<<<g, kPlanBlock, 0, st>>>(d_a_off, d_b_off, P, nb, ctl);
     plan_bins_scan<<<1, 64, 0, st>>>(nb, ctl);
     plan_scatter<<<g, kPlanBlock, 0, st>>>(d_a_off, d_b_off, P, nb, ctl, order);
-    const int blocks = device_cu_count() * plan->opts.waves_per_cu / kWavesPerBlock;
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev0, st));
-    pairs_batched_kernel<<<blocks, kPairBlock, 0, st>>>(d_a_off, d_a, d_b_off, d_b, P, order, ctl,
-                                                        d_sizes, plan->cfg);
+    launch_batched(plan->opts, plan->cfg, d_a_off, d_a, d_b_off, d_b, P, order, ctl, d_sizes, st);
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev1, st));
   }
   RS_HIP(hipGetLastError());
@@ -347,9 +609,7 @@ rs_status pairs_launch_ordered(const rs_pairs_opts* opts, const int64_t* d_a_off
   PairsConst cfg{};
   RS_TRY(validate_opts(opts, &cfg));
   plan_reset<<<1, 64, 0, st>>>(d_ctl);
-  const int blocks = device_cu_count() * opts->waves_per_cu / kWavesPerBlock;
-  pairs_batched_kernel<<<blocks, kPairBlock, 0, st>>>(d_a_off, d_a, d_b_off, d_b, count, d_order,
-                                                      d_ctl, d_sizes, cfg);
+  launch_batched(*opts, cfg, d_a_off, d_a, d_b_off, d_b, count, d_order, d_ctl, d_sizes, st);
   RS_HIP(hipGetLastError());
   return RS_OK;
 }
@@ -364,10 +624,12 @@ void rs_pairs_default_opts(rs_pairs_opts* o) {
   memset(o, 0, sizeof(*o));
   o->num_bins = RS_PAIRS_MAX_BINS;
   o->waves_per_cu = 16;
+  o->lds_slots_per_wave = 2048;
   for (int i = 0; i < RS_PAIRS_MAX_BINS; ++i) {
-    // alpha 0.25, b 2: L = nextpow2 > 2*chunk, so a 511-element chunk fills
-    // the 2048-slot region at load 1/4.
-    o->bins[i] = rs_pairs_bin{0.25f, 2, 511, 8};
+    // alpha 1/8, b 4: L = nextpow2 > 2*chunk, so a 255-element chunk fills
+    // the 2048-slot region at <= 1/2 key per 4-slot bucket, where a home
+    // bucket overflows (forcing the walking lookup) in < 10% of chunks.
+    o->bins[i] = rs_pairs_bin{0.125f, 4, 255, 1};
   }
 }
 

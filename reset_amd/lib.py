@@ -41,7 +41,7 @@ class rs_pairs_bin(C.Structure):
 
 class rs_pairs_opts(C.Structure):
     _fields_ = [("num_bins", C.c_int32), ("bins", rs_pairs_bin * PAIRS_MAX_BINS),
-                ("waves_per_cu", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("lds_slots_per_wave", C.c_int32)]
 
 
 class rs_ir_completion(C.Structure):

@@ -120,8 +120,9 @@ def test_graph_replays_match_eager(api):
     plan.close()
 
 
-@pytest.mark.parametrize("alpha,b,chunk,unroll", [(0.2, 5, 255, 8), (0.5, 2, 1000, 4), (0.9, 1, 1000, 2),
-                                                  (0.25, 2, 511, 1), (0.12, 4, 200, 8)])
+@pytest.mark.parametrize("alpha,b,chunk,unroll", [(0.2, 4, 255, 4), (0.5, 2, 1000, 4), (0.9, 1, 1000, 2),
+                                                  (0.25, 2, 511, 1), (0.12, 4, 200, 2), (0.5, 4, 1023, 2),
+                                                  (0.95, 1, 64, 1)])
 def test_batched_bin_parameters(api, alpha, b, chunk, unroll):
     from reset_amd import inputs
 
