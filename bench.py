@@ -50,6 +50,14 @@ def bench_pairs(args):
     from reset_amd import api, inputs
 
     P = args.pairs
+    if args.copy_probe_mb:
+        # a plain device copy of known size, profiled beside the pair kernel to
+        # calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        _sync_all()
+        del src, dst
     t0 = time.time()
     batch = inputs.make_pair_batch(P, seed=SEED)
     _sync_all()
@@ -259,6 +267,8 @@ def main(argv=None):
     ap.add_argument("--plain-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain copy of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

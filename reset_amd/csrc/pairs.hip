@@ -116,15 +116,18 @@ __global__ void __launch_bounds__(kPlanBlock) plan_scatter(const int64_t* __rest
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// Insert; returns 0 if v landed in its home bucket, 1 if it was displaced to
-// a later bucket (the home bucket was full), -1 if the table had no free slot.
+// Insert; returns the slot index taken (and sets `moved` if v had to leave
+// its home bucket because it was full), or -1 if the table had no free slot.
 template <int B>
-__device__ __forceinline__ int lds_insert(int32_t* tab, uint32_t L, int32_t v) {
+__device__ __forceinline__ int32_t lds_insert(int32_t* tab, uint32_t L, int32_t v, bool& moved) {
   uint32_t bucket = (uint32_t)v & (L - 1u);
   for (uint32_t step = 0; step < L; ++step) {
 #pragma unroll
     for (int k = 0; k < B; ++k)
-      if (atomicCAS(&tab[bucket * B + k], kEmpty, v) == kEmpty) return step == 0 ? 0 : 1;
+      if (atomicCAS(&tab[bucket * B + k], kEmpty, v) == kEmpty) {
+        moved |= step != 0;
+        return (int32_t)(bucket * B + k);
+      }
     bucket = (bucket + 1u) & (L - 1u);
   }
   return -1;
@@ -263,27 +266,35 @@ __device__ __forceinline__ void load_block(const int32_t* __restrict__ G, int32_
   }
 }
 
-// Fills the wave's LDS table with chunk [c0, c0+cl) of the build side, held in
-// registers (sv[r] = S[c0 + r*64 + lane]).  Returns whether some key had to
-// leave its home bucket.
+// Inserts chunk [c0, c0+cl) of the build side, held in registers
+// (sv[r] = S[c0 + r*64 + lane]), into the wave's LDS table, which is all
+// EMPTY on entry; pos[r] records the slot each key took, so clear_chunk can
+// restore the all-EMPTY state by touching only those slots.  Returns whether
+// some key had to leave its home bucket.
 template <int B, int CPL>
 __device__ __forceinline__ bool build_chunk(int32_t* tab, uint32_t L, const int32_t (&sv)[CPL],
-                                            int32_t cl, bool& bad) {
+                                            int32_t cl, int32_t (&pos)[CPL], bool& bad) {
   const int lane = __lane_id();
-  const int32_t slots = (int32_t)L * B;
-  for (int32_t s4 = lane; s4 < (slots >> 2); s4 += kWave)
-    *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
-  __builtin_amdgcn_wave_barrier();
   bool moved = false;
 #pragma unroll
-  for (int r = 0; r < CPL; ++r)
+  for (int r = 0; r < CPL; ++r) {
+    pos[r] = -1;
     if (r * kWave + lane < cl) {
-      const int rc = lds_insert<B>(tab, L, sv[r]);
-      bad |= rc < 0;
-      moved |= rc != 0;
+      pos[r] = lds_insert<B>(tab, L, sv[r], moved);
+      bad |= pos[r] < 0;
     }
+  }
   __builtin_amdgcn_wave_barrier();
   return __any(moved);
+}
+
+template <int CPL>
+__device__ __forceinline__ void clear_chunk(int32_t* tab, const int32_t (&pos)[CPL]) {
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int r = 0; r < CPL; ++r)
+    if (pos[r] >= 0) tab[pos[r]] = kEmpty;
+  __builtin_amdgcn_wave_barrier();
 }
 
 template <int CPL>
@@ -348,7 +359,8 @@ __device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restric
   GBlock<U> cur;
   load_block<U>(G, pos, 0, ng, cur);
   uint32_t L = (uint32_t)calc_length(cl, alpha, B);
-  bool displaced = build_chunk<B, CPL>(tab, L, sv, cl, bad);
+  int32_t spos[CPL];
+  bool displaced = build_chunk<B, CPL>(tab, L, sv, cl, spos, bad);
   while (true) {
     GBlock<U> nxt;
     const bool more = pos + kBlock < ng;
@@ -379,13 +391,14 @@ __device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restric
         nhi = n0 + nl < ns ? (uint32_t)S[n0 + nl] : 0xFFFFFFFFu;
       }
       L = (uint32_t)calc_length(cl, alpha, B);
-      __builtin_amdgcn_wave_barrier();
-      displaced = build_chunk<B, CPL>(tab, L, sv, cl, bad);
+      clear_chunk<CPL>(tab, spos);
+      displaced = build_chunk<B, CPL>(tab, L, sv, cl, spos, bad);
     }
     if (!more) break;
     pos += kBlock;
     cur = nxt;
   }
+  clear_chunk<CPL>(tab, spos);
   __builtin_amdgcn_wave_barrier();
   return hits;
 }
@@ -405,6 +418,10 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
   // wave-uniform: pair index, offsets and loop bounds stay in SGPRs
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int32_t* tab = lds_dyn + wid * slots_per_wave;
+  // the region starts all EMPTY; every pair leaves it that way
+  for (int32_t s4 = lane; s4 < (slots_per_wave >> 2); s4 += kWave)
+    *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
+  __builtin_amdgcn_wave_barrier();
   const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
   int64_t idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
   bool bad = false;
