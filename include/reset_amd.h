@@ -152,14 +152,22 @@ rs_status rs_sss_verify_async(const rs_tables* t, int32_t layout, const int32_t*
 
 /* Prefix-filter self-join of a set collection at Jaccard >= p/q: global
  * token order by ascending frequency, prefix candidates with the size filter,
- * batched hash verification with the exact rule.  The result is the sorted
- * list of pairs (i < j) as int64 keys (i << 32 | j), held by the handle. */
+ * verification of every candidate by the batched LDS hash kernel (indexed
+ * pairs, rs_pairs_intersect_indexed_async) and the exact rule.  Sets must be
+ * ascending with distinct tokens; load_factor/bucket_size are accepted for
+ * interface compatibility (the batched kernel sizes its own LDS tables).
+ * The result is the sorted list of pairs (i < j) as int64 keys
+ * (i << 32 | j), held by the handle. */
 typedef struct rs_sss_result rs_sss_result;
 rs_status rs_sss_selfjoin(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
                           int32_t p, int32_t q, float load_factor, int32_t bucket_size,
                           rs_stream stream, rs_sss_result** out);
 rs_status rs_sss_result_info(const rs_sss_result* r, int64_t* h_num_pairs,
                              int64_t* h_num_candidates, const int64_t** d_pairs);
+/* Work of the verification stage: sum over candidates of the smaller set's
+ * size, and of both sizes (the elements the batched kernel streams). */
+rs_status rs_sss_result_work(const rs_sss_result* r, int64_t* h_verify_probes,
+                             int64_t* h_verify_elements);
 /* Stage timings of the last self-join (ms): token order, candidates, verify. */
 rs_status rs_sss_result_timings(const rs_sss_result* r, float* h_ms3);
 void rs_sss_result_free(rs_sss_result* r);
@@ -259,6 +267,13 @@ void rs_pairs_plan_free(rs_pairs_plan* p);
 rs_status rs_pairs_intersect_batched_async(rs_pairs_plan* plan, const int64_t* d_a_off,
                                            const int32_t* d_a, const int64_t* d_b_off,
                                            const int32_t* d_b, int64_t num_pairs, int32_t* d_sizes,
+                                           rs_stream stream);
+/* The batched form over pairs of one set family: pair i is (S_{u[i]}, S_{v[i]})
+ * of the CSR (d_set_offsets, d_set_elements) -- e.g. the candidate list of a
+ * similarity join.  Same plan, kernel and requirements as above. */
+rs_status rs_pairs_intersect_indexed_async(rs_pairs_plan* plan, const int32_t* d_set_offsets,
+                                           const int32_t* d_set_elements, const int32_t* d_u,
+                                           const int32_t* d_v, int64_t num_pairs, int32_t* d_sizes,
                                            rs_stream stream);
 /* Capture the batched sequence for these buffers into a hipGraph held by the
  * plan, then replay it with rs_pairs_graph_launch. */

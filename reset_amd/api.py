@@ -204,6 +204,34 @@ def ir_result_offsets(index_offsets, query, query_offsets):
     return roff
 
 
+# ---------------------------------------------------------------- set similarity
+def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, stream=None):
+    """Prefix-filter self-join at Jaccard >= p/q with hash verification.
+    -> dict(pairs: int64 tensor of i<<32|j ascending (device), candidates,
+            verify_probes, ms: [order, candidates, verify])"""
+    _need_cuda(offsets, elements)
+    h = C.c_void_p()
+    check(load().rs_sss_selfjoin(ptr(offsets), offsets.numel() - 1, ptr(elements), int(p), int(q),
+                                 float(load_factor), int(bucket_size), stream_handle(stream), C.byref(h)))
+    try:
+        npairs, ncand, dptr = C.c_int64(0), C.c_int64(0), C.c_void_p()
+        check(load().rs_sss_result_info(h, C.byref(npairs), C.byref(ncand), C.byref(dptr)))
+        probes, elems = C.c_int64(0), C.c_int64(0)
+        check(load().rs_sss_result_work(h, C.byref(probes), C.byref(elems)))
+        ms = (C.c_float * 3)()
+        check(load().rs_sss_result_timings(h, ms))
+        pairs = torch.empty(max(npairs.value, 1), dtype=torch.int64, device=offsets.device)
+        if npairs.value:
+            torch.cuda.synchronize()
+            rc = _hip().hipMemcpy(C.c_void_p(pairs.data_ptr()), dptr, C.c_size_t(8 * npairs.value), 3)
+            if rc != 0:
+                raise RuntimeError(f"hipMemcpy failed: {rc}")
+        return dict(pairs=pairs[:npairs.value], candidates=int(ncand.value), verify_probes=int(probes.value),
+                    verify_elements=int(elems.value), ms=[float(x) for x in ms])
+    finally:
+        load().rs_sss_result_free(h)
+
+
 # ---------------------------------------------------------------- pairs
 def default_pairs_opts():
     o = _L.rs_pairs_opts()

@@ -40,6 +40,48 @@ __device__ __forceinline__ int pair_bin(int64_t total, int num_bins) {
   return bin >= num_bins ? num_bins - 1 : bin;
 }
 
+// ---------------------------------------------------------------- pair sources
+// Pair i is (A_i, B_i).  CSR form: A_i = a[a_off[i] .. a_off[i+1]), B_i the
+// same in b.  Indexed form: A_i = S_{u[i]}, B_i = S_{v[i]} of one set family
+// (off, el) -- the candidate lists of a set-similarity join.
+struct PairSrcCsr {
+  const int64_t* a_off;
+  const int32_t* a;
+  const int64_t* b_off;
+  const int32_t* b;
+  __device__ __forceinline__ void get(int64_t i, const int32_t*& A, int32_t& na, const int32_t*& B,
+                                      int32_t& nb) const {
+    const int64_t a0 = a_off[i], b0 = b_off[i];
+    A = a + a0;
+    na = (int32_t)(a_off[i + 1] - a0);
+    B = b + b0;
+    nb = (int32_t)(b_off[i + 1] - b0);
+  }
+  __device__ __forceinline__ int64_t total(int64_t i) const {
+    return (a_off[i + 1] - a_off[i]) + (b_off[i + 1] - b_off[i]);
+  }
+};
+
+struct PairSrcIdx {
+  const int32_t* off;
+  const int32_t* el;
+  const int32_t* u;
+  const int32_t* v;
+  __device__ __forceinline__ void get(int64_t i, const int32_t*& A, int32_t& na, const int32_t*& B,
+                                      int32_t& nb) const {
+    const int32_t x = u[i], y = v[i];
+    const int32_t a0 = off[x], b0 = off[y];
+    A = el + a0;
+    na = off[x + 1] - a0;
+    B = el + b0;
+    nb = off[y + 1] - b0;
+  }
+  __device__ __forceinline__ int64_t total(int64_t i) const {
+    const int32_t x = u[i], y = v[i];
+    return (int64_t)(off[x + 1] - off[x]) + (off[y + 1] - off[y]);
+  }
+};
+
 // ---------------------------------------------------------------- plan
 // Layout of the plan's control block (int32): [0..kMaxBins) histogram,
 // [kMaxBins..2kMaxBins) cursors, [2kMaxBins] work counter, [2kMaxBins+1]
@@ -54,16 +96,14 @@ __global__ void plan_reset(int32_t* ctl) {
   if (threadIdx.x < kCtlWords - 1) ctl[threadIdx.x] = 0;  // keeps the error word
 }
 
-__global__ void plan_hist(const int64_t* __restrict__ a_off, const int64_t* __restrict__ b_off,
-                          int64_t P, int num_bins, int32_t* ctl) {
+template <class Src>
+__global__ void plan_hist(const Src src, int64_t P, int num_bins, int32_t* ctl) {
   __shared__ int32_t h[RS_PAIRS_MAX_BINS];
   if (threadIdx.x < RS_PAIRS_MAX_BINS) h[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t tot = (a_off[i + 1] - a_off[i]) + (b_off[i + 1] - b_off[i]);
-    atomicAdd(&h[pair_bin(tot, num_bins)], 1);
-  }
+       i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[pair_bin(src.total(i), num_bins)], 1);
   __syncthreads();
   if (threadIdx.x < num_bins && h[threadIdx.x]) atomicAdd(&ctl[kCtlHist + threadIdx.x], h[threadIdx.x]);
 }
@@ -81,10 +121,9 @@ __global__ void plan_bins_scan(int num_bins, int32_t* ctl) {
 
 // Block-aggregated scatter: ranks inside the block come from LDS atomics,
 // then one global atomic per (block, bin) reserves the block's range.
-__global__ void __launch_bounds__(kPlanBlock) plan_scatter(const int64_t* __restrict__ a_off,
-                                                           const int64_t* __restrict__ b_off,
-                                                           int64_t P, int num_bins, int32_t* ctl,
-                                                           int32_t* __restrict__ order) {
+template <class Src>
+__global__ void __launch_bounds__(kPlanBlock) plan_scatter(const Src src, int64_t P, int num_bins,
+                                                           int32_t* ctl, int32_t* __restrict__ order) {
   __shared__ int32_t cnt[RS_PAIRS_MAX_BINS];
   __shared__ int32_t base[RS_PAIRS_MAX_BINS];
   for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < P; i0 += (int64_t)gridDim.x * blockDim.x) {
@@ -93,8 +132,7 @@ __global__ void __launch_bounds__(kPlanBlock) plan_scatter(const int64_t* __rest
     const int64_t i = i0 + threadIdx.x;
     int bin = 0, rank = 0;
     if (i < P) {
-      const int64_t tot = (a_off[i + 1] - a_off[i]) + (b_off[i + 1] - b_off[i]);
-      bin = pair_bin(tot, num_bins);
+      bin = pair_bin(src.total(i), num_bins);
       rank = atomicAdd(&cnt[bin], 1);
     }
     __syncthreads();
@@ -106,7 +144,6 @@ __global__ void __launch_bounds__(kPlanBlock) plan_scatter(const int64_t* __rest
   }
 }
 
-// ---------------------------------------------------------------- LDS table
 // ---------------------------------------------------------------- LDS tables
 // A wave's transient table lives in its private LDS region, bucket-major:
 // slot k of bucket j at tab[j*B + k], so one B-slot bucket is one 4/8/16-byte
@@ -407,12 +444,10 @@ __device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restric
 // of the size-sorted order (largest first), so every wave gets a near-equal
 // mix and no global atomic is needed; the next pair's offsets are fetched
 // before the current pair is processed.
-template <int B, int U, int CPL>
+template <int B, int U, int CPL, class Src>
 __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
-    const int64_t* __restrict__ a_off, const int32_t* __restrict__ a,
-    const int64_t* __restrict__ b_off, const int32_t* __restrict__ b, int64_t P,
-    const int32_t* __restrict__ order, int32_t* ctl, int32_t* __restrict__ sizes,
-    const PairsConst cfg, int32_t slots_per_wave) {
+    const Src src, int64_t P, const int32_t* __restrict__ order, int32_t* ctl,
+    int32_t* __restrict__ sizes, const PairsConst cfg, int32_t slots_per_wave) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_dyn[];
   const int lane = __lane_id();
   // wave-uniform: pair index, offsets and loop bounds stay in SGPRs
@@ -426,23 +461,27 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
   int64_t idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
   bool bad = false;
   int32_t i = 0;
-  int64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+  const int32_t* A = nullptr;
+  const int32_t* Bp = nullptr;
+  int32_t na = 0, nb = 0;
   if (idx < P) {
     i = order[idx];
-    a0 = a_off[i]; a1 = a_off[i + 1]; b0 = b_off[i]; b1 = b_off[i + 1];
+    src.get(i, A, na, Bp, nb);
   }
   for (; idx < P; idx += W) {
     // prefetch the next pair's metadata
     const int64_t nidx = idx + W;
     int32_t ni = 0;
-    int64_t na0 = 0, na1 = 0, nb0 = 0, nb1 = 0;
+    const int32_t* nA = nullptr;
+    const int32_t* nB = nullptr;
+    int32_t nna = 0, nnb = 0;
     if (nidx < P) {
       ni = order[nidx];
-      na0 = a_off[ni]; na1 = a_off[ni + 1]; nb0 = b_off[ni]; nb1 = b_off[ni + 1];
+      src.get(ni, nA, nna, nB, nnb);
     }
-    int32_t ns = (int32_t)(a1 - a0), ng = (int32_t)(b1 - b0);
-    const int32_t* S = a + a0;
-    const int32_t* Gp = b + b0;
+    int32_t ns = na, ng = nb;
+    const int32_t* S = A;
+    const int32_t* Gp = Bp;
     if (ns > ng) {
       const int32_t* t = S;
       S = Gp;
@@ -458,7 +497,11 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     }
     const uint32_t total = wave_sum(hits);
     if (lane == 0) sizes[i] = (int32_t)total;
-    i = ni; a0 = na0; a1 = na1; b0 = nb0; b1 = nb1;
+    i = ni;
+    A = nA;
+    Bp = nB;
+    na = nna;
+    nb = nnb;
   }
   if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
 }
@@ -559,23 +602,22 @@ static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   return RS_OK;
 }
 
-template <int B, int U>
-static void launch_bu(int cpl, int blocks, size_t lds, const int64_t* a_off, const int32_t* a,
-                      const int64_t* b_off, const int32_t* b, int64_t P, const int32_t* order,
-                      int32_t* ctl, int32_t* sizes, const PairsConst& cfg, int32_t spw,
-                      hipStream_t st) {
+template <int B, int U, class Src>
+static void launch_bu(int cpl, int blocks, size_t lds, const Src& src, int64_t P,
+                      const int32_t* order, int32_t* ctl, int32_t* sizes, const PairsConst& cfg,
+                      int32_t spw, hipStream_t st) {
   if (cpl <= 4)
-    pairs_batched_kernel<B, U, 4><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 4, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
   else if (cpl <= 8)
-    pairs_batched_kernel<B, U, 8><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 8, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
   else
-    pairs_batched_kernel<B, U, 16><<<blocks, kPairBlock, lds, st>>>(a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 16, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
 }
 
 // Launch-level shape: bucket size B and unroll U are the same in every bin;
 // CPL (build-side registers per lane) covers the largest chunk of any bin.
-static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const int64_t* a_off,
-                           const int32_t* a, const int64_t* b_off, const int32_t* b, int64_t P,
+template <class Src>
+static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const Src& src, int64_t P,
                            const int32_t* order, int32_t* ctl, int32_t* sizes, hipStream_t st) {
   const int blocks = device_cu_count() * o.waves_per_cu / kWavesPerBlock;
   int max_chunk = 1;
@@ -584,8 +626,7 @@ static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const 
   const int32_t spw = o.lds_slots_per_wave;
   const size_t lds = sizeof(int32_t) * (size_t)spw * kWavesPerBlock;
   const int B = o.bins[0].bucket_size, U = o.bins[0].unroll;
-#define RS_LAUNCH_BU(b_, u_) \
-  launch_bu<b_, u_>(cpl, blocks, lds, a_off, a, b_off, b, P, order, ctl, sizes, cfg, spw, st)
+#define RS_LAUNCH_BU(b_, u_) launch_bu<b_, u_, Src>(cpl, blocks, lds, src, P, order, ctl, sizes, cfg, spw, st)
   if (B == 4) {
     if (U == 1) RS_LAUNCH_BU(4, 1); else if (U == 2) RS_LAUNCH_BU(4, 2); else RS_LAUNCH_BU(4, 4);
   } else if (B == 2) {
@@ -596,24 +637,30 @@ static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const 
 #undef RS_LAUNCH_BU
 }
 
-static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,
-                               const int64_t* d_b_off, const int32_t* d_b, int64_t P,
-                               int32_t* d_sizes, hipStream_t st) {
+template <class Src>
+static rs_status pairs_enqueue(rs_pairs_plan* plan, const Src& src, int64_t P, int32_t* d_sizes,
+                               hipStream_t st) {
   int32_t* ctl = plan->ctl.as<int32_t>();
   int32_t* order = plan->order.as<int32_t>();
   const int nb = plan->cfg.num_bins;
   plan_reset<<<1, 64, 0, st>>>(ctl);
   if (P > 0) {
     const int g = grid_for(P, kPlanBlock, 4);
-    plan_hist<<<g, kPlanBlock, 0, st>>>(d_a_off, d_b_off, P, nb, ctl);
+    plan_hist<Src><<<g, kPlanBlock, 0, st>>>(src, P, nb, ctl);
     plan_bins_scan<<<1, 64, 0, st>>>(nb, ctl);
-    plan_scatter<<<g, kPlanBlock, 0, st>>>(d_a_off, d_b_off, P, nb, ctl, order);
+    plan_scatter<Src><<<g, kPlanBlock, 0, st>>>(src, P, nb, ctl, order);
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev0, st));
-    launch_batched(plan->opts, plan->cfg, d_a_off, d_a, d_b_off, d_b, P, order, ctl, d_sizes, st);
+    launch_batched<Src>(plan->opts, plan->cfg, src, P, order, ctl, d_sizes, st);
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev1, st));
   }
   RS_HIP(hipGetLastError());
   return RS_OK;
+}
+
+static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,
+                               const int64_t* d_b_off, const int32_t* d_b, int64_t P,
+                               int32_t* d_sizes, hipStream_t st) {
+  return pairs_enqueue<PairSrcCsr>(plan, PairSrcCsr{d_a_off, d_a, d_b_off, d_b}, P, d_sizes, st);
 }
 
 namespace rs {
@@ -626,7 +673,8 @@ rs_status pairs_launch_ordered(const rs_pairs_opts* opts, const int64_t* d_a_off
   PairsConst cfg{};
   RS_TRY(validate_opts(opts, &cfg));
   plan_reset<<<1, 64, 0, st>>>(d_ctl);
-  launch_batched(*opts, cfg, d_a_off, d_a, d_b_off, d_b, count, d_order, d_ctl, d_sizes, st);
+  launch_batched<PairSrcCsr>(*opts, cfg, PairSrcCsr{d_a_off, d_a, d_b_off, d_b}, count, d_order, d_ctl,
+                             d_sizes, st);
   RS_HIP(hipGetLastError());
   return RS_OK;
 }
@@ -687,6 +735,19 @@ rs_status rs_pairs_intersect_batched_async(rs_pairs_plan* plan, const int64_t* d
              "rs_pairs_intersect_batched_async: %lld pairs exceed the plan's %lld",
              (long long)num_pairs, (long long)plan->max_pairs);
   return pairs_enqueue(plan, d_a_off, d_a, d_b_off, d_b, num_pairs, d_sizes, as_stream(stream));
+}
+
+rs_status rs_pairs_intersect_indexed_async(rs_pairs_plan* plan, const int32_t* d_set_offsets,
+                                           const int32_t* d_set_elements, const int32_t* d_u,
+                                           const int32_t* d_v, int64_t num_pairs, int32_t* d_sizes,
+                                           rs_stream stream) {
+  RS_REQUIRE(plan && d_set_offsets && d_set_elements && (num_pairs == 0 || (d_u && d_v && d_sizes)),
+             "rs_pairs_intersect_indexed_async: null argument");
+  RS_REQUIRE(num_pairs >= 0 && num_pairs <= plan->max_pairs,
+             "rs_pairs_intersect_indexed_async: %lld pairs exceed the plan's %lld",
+             (long long)num_pairs, (long long)plan->max_pairs);
+  return pairs_enqueue<PairSrcIdx>(plan, PairSrcIdx{d_set_offsets, d_set_elements, d_u, d_v}, num_pairs,
+                                   d_sizes, as_stream(stream));
 }
 
 rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,

@@ -1,0 +1,466 @@
+// sss_join.hip -- prefix-filter set-similarity self-join (Jaccard >= p/q)
+// feeding the batched hash verification of sss.hip.
+//
+// Global token order: ascending frequency, ties by token id.  With every set
+// written in that order, two sets with J >= t share a token within their
+// prefixes of length s - ceil(t*s) + 1 (Chaudhuri et al., ICDE 2006), and
+// J >= t also needs p*max(|A|,|B|) <= q*min(|A|,|B|) (size filter).
+// Stages (all on the device):
+//   1. token frequencies (radix sort of the tokens, run boundaries)
+//   2. rank[token] by (frequency, id) (radix sort of 64-bit keys)
+//   3. each set's tokens as ranks, sorted per set (segmented radix sort)
+//   4. prefix entries (rank, size, set) sorted; per entry, its partners are
+//      the later entries of the same rank whose size passes the size filter
+//      (a contiguous window, found by binary search)
+//   5. candidate keys (i << 32 | j), radix sort + unique
+//   6. hash verification of every candidate (exact integer rule), compaction
+// Limits: < 2^21 sets, sets < 2^21 tokens, token ids < 2^22.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include "sss_verify.hpp"
+
+struct rs_sss_result {
+  rs::DevBuf pairs;  // int64 keys i << 32 | j, ascending
+  int64_t num_pairs = 0;
+  int64_t num_candidates = 0;
+  int64_t verify_probes = 0;    // sum over candidates of min(|S_i|, |S_j|)
+  int64_t verify_elements = 0;  // sum over candidates of |S_i| + |S_j|
+  float ms[3] = {0, 0, 0};
+};
+
+namespace rs {
+
+constexpr int kJoinBlock = 256;
+
+__global__ void max_token_kernel(const int32_t* __restrict__ x, int64_t n, int32_t* out,
+                                 int32_t* err) {
+  int32_t m = -1;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = x[i];
+    bad |= v < 0;
+    m = v > m ? v : m;
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const int32_t o = __shfl_xor(m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if (__lane_id() == 0) atomicMax(out, m);
+  if (__any(bad) && __lane_id() == 0) atomicOr(err, kErrBadInput);
+}
+
+// freq[t] from the sorted token array: run [s, e) of value t gives e - s.
+__global__ void run_freq_kernel(const int32_t* __restrict__ sorted, int64_t n,
+                                int32_t* __restrict__ freq) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = sorted[i];
+    if (i + 1 == n || sorted[i + 1] != v) {
+      // i is the end of v's run; find its start by the previous run end
+      int64_t lo = 0, hi = i;  // first index with value v in [0, i]
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (sorted[mid] < v) lo = mid + 1; else hi = mid;
+      }
+      freq[v] = (int32_t)(i + 1 - lo);
+    }
+  }
+}
+
+__global__ void order_keys_kernel(const int32_t* __restrict__ freq, int64_t V,
+                                  uint64_t* __restrict__ keys) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < V;
+       t += (int64_t)gridDim.x * blockDim.x)
+    keys[t] = ((uint64_t)(uint32_t)freq[t] << 32) | (uint64_t)t;
+}
+
+__global__ void rank_kernel(const uint64_t* __restrict__ sorted_keys, int64_t V,
+                            int32_t* __restrict__ rank) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V;
+       i += (int64_t)gridDim.x * blockDim.x)
+    rank[(int64_t)(sorted_keys[i] & 0xFFFFFFFFull)] = (int32_t)i;
+}
+
+__global__ void map_rank_kernel(const int32_t* __restrict__ tok, int64_t n,
+                                const int32_t* __restrict__ rank, int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = rank[tok[i]];
+}
+
+__device__ __forceinline__ int32_t prefix_len(int32_t s, int32_t p, int32_t q) {
+  int64_t pl = (int64_t)s - ((int64_t)p * s + q - 1) / q + 1;
+  if (pl > s) pl = s;
+  return pl < 0 ? 0 : (int32_t)pl;
+}
+
+__global__ void prefix_count_kernel(const int32_t* __restrict__ off, int64_t n, int32_t p,
+                                    int32_t q, int32_t* __restrict__ pl, int32_t* err) {
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = off[i + 1] - off[i];
+    bad |= s < 0 || s >= (1 << 21);
+    pl[i] = bad ? 0 : prefix_len(s, p, q);
+  }
+  if (__any(bad) && __lane_id() == 0) atomicOr(err, kErrBadInput);
+}
+
+constexpr int kRankShift = 42, kSizeShift = 21;
+constexpr uint64_t kField21 = (1ull << 21) - 1;
+
+__global__ void prefix_emit_kernel(const int32_t* __restrict__ off, int64_t n,
+                                   const int32_t* __restrict__ ranked,
+                                   const int64_t* __restrict__ poff, int32_t p, int32_t q,
+                                   uint64_t* __restrict__ entries) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = off[i + 1] - off[i];
+    const int32_t pl = prefix_len(s, p, q);
+    for (int32_t k = 0; k < pl; ++k)
+      entries[poff[i] + k] = ((uint64_t)(uint32_t)ranked[off[i] + k] << kRankShift) |
+                             ((uint64_t)s << kSizeShift) | (uint64_t)i;
+  }
+}
+
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo, int64_t hi,
+                                                   uint64_t key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Partners of entry x: later entries y of the same rank with
+// p * size_y <= q * size_x (entries are sorted by rank, then size).
+__global__ void cand_count_kernel(const uint64_t* __restrict__ ent, int64_t m, int32_t p,
+                                  int32_t q, int64_t* __restrict__ cnt) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = ent[x];
+    const uint64_t rank = k >> kRankShift;
+    const uint64_t sz = (k >> kSizeShift) & kField21;
+    uint64_t smax = ((uint64_t)q * sz) / (uint64_t)p;
+    if (smax > kField21) smax = kField21;
+    const uint64_t bound = (rank << kRankShift) | (smax << kSizeShift) | kField21;
+    const int64_t end = lower_bound_u64(ent, x + 1, m, bound + 1);
+    cnt[x] = end - x - 1;
+  }
+}
+
+__global__ void cand_emit_kernel(const uint64_t* __restrict__ ent, int64_t m,
+                                 const int64_t* __restrict__ coff, uint64_t* __restrict__ cand) {
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < m;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = coff[x + 1] - coff[x];
+    const uint64_t ix = ent[x] & kField21;
+    for (int64_t j = 0; j < c; ++j) {
+      const uint64_t iy = ent[x + 1 + j] & kField21;
+      cand[coff[x] + j] = ix < iy ? ((ix << 32) | iy) : ((iy << 32) | ix);
+    }
+  }
+}
+
+__global__ void split_pairs_kernel(const uint64_t* __restrict__ cand, int64_t n,
+                                   int32_t* __restrict__ u, int32_t* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    u[i] = (int32_t)(cand[i] >> 32);
+    v[i] = (int32_t)(cand[i] & 0xFFFFFFFFull);
+  }
+}
+
+// Exact rule on the pair sizes r: flag iff q*r >= p*(|S_u|+|S_v|-r) > ... ;
+// also accumulates sum(min) and sum(|S_u|+|S_v|) for the work accounting.
+__global__ void jaccard_flags_kernel(const int32_t* __restrict__ off, const int32_t* __restrict__ u,
+                                     const int32_t* __restrict__ v, const int32_t* __restrict__ r,
+                                     int64_t n, int32_t p, int32_t q, int32_t* __restrict__ flags,
+                                     unsigned long long* work) {
+  unsigned long long smin = 0, ssum = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t su = off[u[i] + 1] - off[u[i]], sv = off[v[i] + 1] - off[v[i]];
+    const int64_t U = su + sv - r[i];
+    flags[i] = U > 0 && (int64_t)q * r[i] >= (int64_t)p * U;
+    smin += (unsigned long long)(su < sv ? su : sv);
+    ssum += (unsigned long long)(su + sv);
+  }
+  smin = wave_sum(smin);
+  ssum = wave_sum(ssum);
+  if (__lane_id() == 0) {
+    atomicAdd(&work[0], smin);
+    atomicAdd(&work[1], ssum);
+  }
+}
+
+__global__ void count_to_i64_kernel(const int64_t* __restrict__ in, int64_t n,
+                                    int64_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+#define RS_ROC(expr)                                                        \
+  do {                                                                      \
+    hipError_t _e = (expr);                                                 \
+    if (_e != hipSuccess) {                                                 \
+      ::rs::set_error("%s failed: %s", #expr, hipGetErrorString(_e));       \
+      return RS_EDEVICE;                                                    \
+    }                                                                       \
+  } while (0)
+
+template <typename T>
+static rs_status sort_keys(const T* in, T* out, int64_t n, unsigned end_bit, DevBuf& tmp,
+                           hipStream_t st) {
+  size_t bytes = 0;
+  RS_ROC(rocprim::radix_sort_keys(nullptr, bytes, in, out, (size_t)n, 0, end_bit, st));
+  if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+  RS_ROC(rocprim::radix_sort_keys(tmp.p, bytes, in, out, (size_t)n, 0, end_bit, st));
+  return RS_OK;
+}
+
+static unsigned bits_for(uint64_t max_value) {
+  unsigned b = 1;
+  while (b < 64 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,
+                          int32_t q, float alpha, int32_t bsz, hipStream_t st, rs_sss_result* R) {
+  RS_REQUIRE(n >= 0 && n < (1ll << 21), "sss self-join: %lld sets exceed 2^21", (long long)n);
+  RS_REQUIRE(p > 0 && q >= p, "sss self-join: threshold p/q must be in (0, 1]");
+  hipEvent_t ev[4];
+  for (auto& e : ev) RS_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  RS_HIP(hipEventRecord(ev[0], st));
+  int32_t E32 = 0;
+  if (n > 0) {
+    RS_HIP(hipMemcpyAsync(&E32, d_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  const int64_t E = E32;
+  DevBuf tmp, scratch;
+  RS_TRY(scratch.alloc(16));
+  int32_t* d_max = scratch.as<int32_t>();
+  RS_HIP(hipMemsetAsync(scratch.p, 0, 16, st));
+  RS_HIP(hipMemsetAsync(d_max, 0xFF, sizeof(int32_t), st));
+  if (E > 0) max_token_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, d_max, d_max + 1);
+  int32_t hm[2];
+  RS_HIP(hipMemcpyAsync(hm, d_max, sizeof(hm), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(!hm[1], "sss self-join: negative token id");
+  const int64_t V = (int64_t)hm[0] + 1;
+  RS_REQUIRE(V < (1ll << 22), "sss self-join: token ids must be < 2^22");
+  // 1-2. frequencies and ranks
+  DevBuf sorted_tok, freq, okeys, okeys_sorted, rank;
+  RS_TRY(sorted_tok.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  RS_TRY(freq.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
+  RS_HIP(hipMemsetAsync(freq.p, 0, sizeof(int32_t) * (size_t)(V > 0 ? V : 1), st));
+  if (E > 0) {
+    RS_TRY(sort_keys<int32_t>(d_el, sorted_tok.as<int32_t>(), E, bits_for((uint64_t)V), tmp, st));
+    run_freq_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(sorted_tok.as<int32_t>(), E, freq.as<int32_t>());
+  }
+  sorted_tok.release();
+  RS_TRY(okeys.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
+  RS_TRY(okeys_sorted.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
+  RS_TRY(rank.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
+  if (V > 0) {
+    order_keys_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(freq.as<int32_t>(), V, okeys.as<uint64_t>());
+    RS_TRY(sort_keys<uint64_t>(okeys.as<uint64_t>(), okeys_sorted.as<uint64_t>(), V,
+                               32 + bits_for((uint64_t)(E > 0 ? E : 1)), tmp, st));
+    rank_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(okeys_sorted.as<uint64_t>(), V, rank.as<int32_t>());
+  }
+  okeys.release();
+  okeys_sorted.release();
+  freq.release();
+  // 3. each set in rank order
+  DevBuf ranked, ranked_sorted;
+  RS_TRY(ranked.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  RS_TRY(ranked_sorted.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  if (E > 0) {
+    map_rank_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, rank.as<int32_t>(), ranked.as<int32_t>());
+    size_t bytes = 0;
+    RS_ROC(rocprim::segmented_radix_sort_keys(nullptr, bytes, ranked.as<int32_t>(),
+                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
+                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
+    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+    RS_ROC(rocprim::segmented_radix_sort_keys(tmp.p, bytes, ranked.as<int32_t>(),
+                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
+                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
+  }
+  ranked.release();
+  rank.release();
+  RS_HIP(hipEventRecord(ev[1], st));
+  // 4. prefix entries
+  DevBuf pl, poff, ent, ent_sorted;
+  RS_TRY(pl.alloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+  RS_TRY(poff.alloc(sizeof(int64_t) * (size_t)(n + 1)));
+  if (n > 0)
+    prefix_count_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, p, q, pl.as<int32_t>(), d_max + 1);
+  RS_TRY(exclusive_scan_i32_to_i64(pl.as<int32_t>(), n, poff.as<int64_t>(), tmp, st));
+  int64_t M = 0;
+  RS_HIP(hipMemcpyAsync(&M, poff.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(hm, d_max, sizeof(hm), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(!hm[1], "sss self-join: a set has 2^21 or more tokens");
+  RS_TRY(ent.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
+  RS_TRY(ent_sorted.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
+  if (n > 0)
+    prefix_emit_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, ranked_sorted.as<int32_t>(),
+                                                            poff.as<int64_t>(), p, q, ent.as<uint64_t>());
+  if (M > 0) RS_TRY(sort_keys<uint64_t>(ent.as<uint64_t>(), ent_sorted.as<uint64_t>(), M, 64, tmp, st));
+  ent.release();
+  ranked_sorted.release();
+  // 5. candidates
+  DevBuf cnt, coff, cand, cand_sorted, uniq, nuniq;
+  RS_TRY(cnt.alloc(sizeof(int64_t) * (size_t)(M > 0 ? M : 1)));
+  RS_TRY(coff.alloc(sizeof(int64_t) * (size_t)(M + 1)));
+  if (M > 0)
+    cand_count_kernel<<<grid_for(M, 256, 8), 256, 0, st>>>(ent_sorted.as<uint64_t>(), M, p, q,
+                                                           cnt.as<int64_t>());
+  RS_TRY(exclusive_scan_i64(cnt.as<int64_t>(), M, coff.as<int64_t>(), tmp, st));
+  int64_t C = 0;
+  RS_HIP(hipMemcpyAsync(&C, coff.as<int64_t>() + M, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  if (C > (1ll << 32)) {
+    set_error("sss self-join: %lld candidate pairs (before dedup) exceed the 2^32 working limit",
+              (long long)C);
+    return RS_ENOMEM;
+  }
+  cnt.release();
+  RS_TRY(cand.alloc(sizeof(uint64_t) * (size_t)(C > 0 ? C : 1)));
+  if (C > 0)
+    cand_emit_kernel<<<grid_for(M, 256, 8), 256, 0, st>>>(ent_sorted.as<uint64_t>(), M,
+                                                          coff.as<int64_t>(), cand.as<uint64_t>());
+  ent_sorted.release();
+  coff.release();
+  int64_t U = 0;
+  RS_TRY(nuniq.alloc(16));
+  if (C > 0) {
+    RS_TRY(cand_sorted.alloc(sizeof(uint64_t) * (size_t)C));
+    RS_TRY(sort_keys<uint64_t>(cand.as<uint64_t>(), cand_sorted.as<uint64_t>(), C, 64, tmp, st));
+    cand.release();
+    RS_TRY(uniq.alloc(sizeof(uint64_t) * (size_t)C));
+    size_t bytes = 0;
+    RS_ROC(rocprim::unique(nullptr, bytes, cand_sorted.as<uint64_t>(), uniq.as<uint64_t>(),
+                           nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
+    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+    RS_ROC(rocprim::unique(tmp.p, bytes, cand_sorted.as<uint64_t>(), uniq.as<uint64_t>(),
+                           nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
+    size_t hu = 0;
+    RS_HIP(hipMemcpyAsync(&hu, nuniq.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    U = (int64_t)hu;
+    cand_sorted.release();
+  }
+  R->num_candidates = U;
+  RS_HIP(hipEventRecord(ev[2], st));
+  // 6. verification: every candidate's |S_i cap S_j| by the batched LDS
+  //    hash kernel (indexed pairs), then the exact rule + compaction
+  int64_t K = 0;
+  if (U > 0) {
+    DevBuf cu, cv, flags, sizes, out_n, work;
+    RS_TRY(cu.alloc(sizeof(int32_t) * (size_t)U));
+    RS_TRY(cv.alloc(sizeof(int32_t) * (size_t)U));
+    RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)U));
+    RS_TRY(sizes.alloc(sizeof(int32_t) * (size_t)U));
+    RS_TRY(work.alloc(16));
+    RS_HIP(hipMemsetAsync(work.p, 0, 16, st));
+    split_pairs_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(uniq.as<uint64_t>(), U, cu.as<int32_t>(),
+                                                           cv.as<int32_t>());
+    rs_pairs_plan* plan = nullptr;
+    RS_TRY(rs_pairs_plan_create(U, nullptr, &plan));
+    struct PGuard {
+      rs_pairs_plan* p;
+      ~PGuard() { rs_pairs_plan_free(p); }
+    } pg{plan};
+    RS_TRY(rs_pairs_intersect_indexed_async(plan, d_off, d_el, cu.as<int32_t>(), cv.as<int32_t>(), U,
+                                            sizes.as<int32_t>(), st));
+    jaccard_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(d_off, cu.as<int32_t>(), cv.as<int32_t>(),
+                                                             sizes.as<int32_t>(), U, p, q,
+                                                             flags.as<int32_t>(),
+                                                             work.as<unsigned long long>());
+    RS_TRY(R->pairs.alloc(sizeof(uint64_t) * (size_t)U));
+    RS_TRY(out_n.alloc(16));
+    size_t bytes = 0;
+    RS_ROC(rocprim::select(nullptr, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
+                           R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
+    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+    RS_ROC(rocprim::select(tmp.p, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
+                           R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
+    size_t hk = 0;
+    unsigned long long hw[2] = {0, 0};
+    RS_HIP(hipMemcpyAsync(&hk, out_n.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(hw, work.p, sizeof(hw), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    K = (int64_t)hk;
+    R->verify_probes = (int64_t)hw[0];
+    R->verify_elements = (int64_t)hw[1];
+    RS_TRY(rs_pairs_plan_status(plan, st));
+  } else {
+    RS_TRY(R->pairs.alloc(16));
+  }
+  R->num_pairs = K;
+  RS_HIP(hipEventRecord(ev[3], st));
+  RS_HIP(hipEventSynchronize(ev[3]));
+  for (int i = 0; i < 3; ++i) RS_HIP(hipEventElapsedTime(&R->ms[i], ev[i], ev[i + 1]));
+  return RS_OK;
+}
+
+}  // namespace rs
+
+using namespace rs;
+
+extern "C" {
+
+rs_status rs_sss_selfjoin(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                          int32_t p, int32_t q, float load_factor, int32_t bucket_size,
+                          rs_stream stream, rs_sss_result** out) {
+  RS_REQUIRE(out && d_offsets && (d_elements || num_sets == 0), "rs_sss_selfjoin: null argument");
+  *out = nullptr;
+  auto* R = new rs_sss_result();
+  rs_status s = selfjoin(d_offsets, num_sets, d_elements, p, q, load_factor, bucket_size,
+                         as_stream(stream), R);
+  if (s != RS_OK) {
+    delete R;
+    return s;
+  }
+  *out = R;
+  return RS_OK;
+}
+
+rs_status rs_sss_result_info(const rs_sss_result* r, int64_t* h_num_pairs,
+                             int64_t* h_num_candidates, const int64_t** d_pairs) {
+  RS_REQUIRE(r, "rs_sss_result_info: null result");
+  if (h_num_pairs) *h_num_pairs = r->num_pairs;
+  if (h_num_candidates) *h_num_candidates = r->num_candidates;
+  if (d_pairs) *d_pairs = r->pairs.as<int64_t>();
+  return RS_OK;
+}
+
+rs_status rs_sss_result_work(const rs_sss_result* r, int64_t* h_verify_probes,
+                             int64_t* h_verify_elements) {
+  RS_REQUIRE(r, "rs_sss_result_work: null result");
+  if (h_verify_probes) *h_verify_probes = r->verify_probes;
+  if (h_verify_elements) *h_verify_elements = r->verify_elements;
+  return RS_OK;
+}
+
+rs_status rs_sss_result_timings(const rs_sss_result* r, float* h_ms3) {
+  RS_REQUIRE(r && h_ms3, "rs_sss_result_timings: null argument");
+  for (int i = 0; i < 3; ++i) h_ms3[i] = r->ms[i];
+  return RS_OK;
+}
+
+void rs_sss_result_free(rs_sss_result* r) { delete r; }
+
+}  // extern "C"

@@ -97,3 +97,88 @@ def make_pair_batch(num_pairs, seed=20261203, device="cuda", **kw):
     k_d = torch.from_numpy(k.astype(np.int32)).to(device)
     a, b = api.gen_pair_sets(a_off_d, b_off_d, k_d, seed)
     return dict(a_off=a_off_d, a=a, b_off=b_off_d, b=b, k=k_d)
+
+
+# ---------------------------------------------------------------- set collections
+def _distinct_rows(counts, universe, draw, gen, device):
+    """CSR of rows with exactly counts[i] distinct values each, sorted: draw the
+    missing number of values per row (draw(n, gen) -> int64 values), dedup the
+    (row, value) keys, repeat until every row is full."""
+    n = counts.numel()
+    keys = torch.empty(0, dtype=torch.int64, device=device)
+    have = torch.zeros(n, dtype=torch.int64, device=device)
+    for _ in range(64):
+        deficit = counts - have
+        tot = int(deficit.sum().item())
+        if tot == 0:
+            break
+        rows = torch.repeat_interleave(torch.arange(n, device=device), deficit)
+        vals = draw(tot, gen)
+        keys = torch.unique(torch.cat([keys, rows * universe + vals]))
+        have = torch.bincount(keys // universe, minlength=n)
+    else:
+        raise RuntimeError("distinct sampling did not converge")
+    off = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    off[1:] = torch.cumsum(have, 0)
+    return off.to(torch.int32), (keys % universe).to(torch.int32)
+
+
+def zipf_set_collection(num_sets=1_000_000, vocab=1_000_000, s_min=10, s_max=2000, z_size=1.2, z_tok=1.0,
+                        seed=20261203, device="cuda"):
+    """Set-similarity input: sizes Zipf(z_size) on [s_min, s_max] (inverse CDF),
+    tokens of rank r drawn with probability proportional to r^-z_tok (token id =
+    r - 1), each set distinct and sorted."""
+    rng = np.random.default_rng(seed)
+    ks = np.arange(s_min, s_max + 1)
+    ps = ks.astype(np.float64) ** -z_size
+    sizes = rng.choice(ks, size=num_sets, p=ps / ps.sum())
+    sizes = np.minimum(sizes, vocab)
+    w = torch.arange(1, vocab + 1, dtype=torch.float64, device=device) ** -z_tok
+    cdf = torch.cumsum(w, 0)
+    cdf /= cdf[-1].clone()
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+
+    def draw(k, g):
+        u = torch.rand(k, generator=g, dtype=torch.float64, device=device)
+        return torch.clamp(torch.searchsorted(cdf, u), max=vocab - 1)
+
+    return _distinct_rows(torch.from_numpy(sizes).to(device), vocab, draw, gen, device)
+
+
+def ir_index(num_docs=10_000_000, num_terms=1_000_000, z=1.1, cap=2_000_000, seed=20261203, device="cuda"):
+    """Inverted index: term of rank r has L_r = clamp(floor(D / r^z), 1, cap)
+    postings, doc ids uniform without replacement in [0, D), sorted."""
+    r = np.arange(1, num_terms + 1, dtype=np.float64)
+    lens = np.clip(np.floor(num_docs / r ** z), 1, cap).astype(np.int64)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+
+    def draw(k, g):
+        return torch.randint(0, num_docs, (k,), generator=g, device=device, dtype=torch.int64)
+
+    return _distinct_rows(torch.from_numpy(lens).to(device), num_docs, draw, gen, device)
+
+
+def ir_queries(index_offsets, num_queries, kmin=2, kmax=8, seed=20261203):
+    """Conjunctive queries: k ~ U{kmin..kmax} distinct terms drawn with
+    probability proportional to their posting-list length, ordered by
+    (document frequency, id) as the reference preprocessing orders them
+    (preprocessor.py:345-349).  -> (terms int32, offsets int32) on the host."""
+    off = np.asarray(index_offsets.cpu().numpy() if hasattr(index_offsets, "cpu") else index_offsets, np.int64)
+    df = np.diff(off)
+    cdf = np.cumsum(df, dtype=np.float64)
+    cdf /= cdf[-1]
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(kmin, kmax + 1, size=num_queries)
+    terms, qoff = [], [0]
+    for k in ks.tolist():
+        got = []
+        while len(got) < k:  # draws proportional to df, repeats rejected
+            t = int(min(np.searchsorted(cdf, rng.random()), len(df) - 1))
+            if t not in got:
+                got.append(t)
+        got.sort(key=lambda t: (df[t], t))
+        terms.extend(got)
+        qoff.append(len(terms))
+    return np.array(terms, np.int32), np.array(qoff, np.int32)

@@ -75,6 +75,7 @@ SIGNATURES = {
     "rs_sss_selfjoin": (_i32, [_p, _i64, _p, _i32, _i32, _f32, _i32, _p, _pp]),
     "rs_sss_result_info": (_i32, [_p, _p, _p, _pp]),
     "rs_sss_result_timings": (_i32, [_p, _p]),
+    "rs_sss_result_work": (_i32, [_p, _p, _p]),
     "rs_sss_result_free": (None, [_p]),
     "rs_ir_query_async": (_i32, [_p, _i32, _p, _p, _p, _i64, _p, _p, _p, _p, _p]),
     "rs_ir_server_create": (_i32, [_p, _i32, _i64, _i32, _i32, _pp]),
@@ -89,6 +90,7 @@ SIGNATURES = {
     "rs_pairs_plan_create": (_i32, [_i64, C.POINTER(rs_pairs_opts), _pp]),
     "rs_pairs_plan_free": (None, [_p]),
     "rs_pairs_intersect_batched_async": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
+    "rs_pairs_intersect_indexed_async": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
     "rs_pairs_graph_capture": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
     "rs_pairs_graph_launch": (_i32, [_p, _p]),
     "rs_pairs_plan_status": (_i32, [_p, _p]),

@@ -146,3 +146,32 @@ def test_bin_parameters_rejected_when_table_exceeds_lds(api):
     opts.bins[3].chunk = 5000
     with pytest.raises(ResetError):
         api.PairsPlan(10, opts)
+
+
+def test_indexed_pairs_match_oracle(api, oracle_lib):
+    """Pairs given as (u, v) indices into one sorted set family (the form a
+    similarity join hands over), including repeated and self pairs."""
+    rng = np.random.default_rng(12)
+    off, cols = G.random_sets(rng, 400, 3000, 1 << 22)
+    P = 5000
+    u = rng.integers(0, 400, P).astype(np.int32)
+    v = rng.integers(0, 400, P).astype(np.int32)
+    v[:50] = u[:50]
+    plan = api.PairsPlan(P)
+    sizes = torch.empty(P, dtype=torch.int32, device="cuda")
+    from reset_amd.api import ptr, stream_handle
+    from reset_amd.lib import check, load
+
+    offd, colsd = torch.from_numpy(off).cuda(), torch.from_numpy(cols).cuda()
+    ud, vd = torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda()
+    check(load().rs_pairs_intersect_indexed_async(plan._h, ptr(offd), ptr(colsd), ptr(ud), ptr(vd), P,
+                                                  ptr(sizes), stream_handle()))
+    plan.status()
+    a_off = np.concatenate([[0], np.cumsum(np.diff(off)[u])]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum(np.diff(off)[v])]).astype(np.int64)
+    A = np.concatenate([cols[off[x]:off[x + 1]] for x in u])
+    B = np.concatenate([cols[off[y]:off[y + 1]] for y in v])
+    want = oracle_lib.pairs_merge(a_off, A, b_off, B)
+    assert np.array_equal(sizes.cpu().numpy(), want)
+    assert np.array_equal(sizes.cpu().numpy()[:50], np.diff(off)[u[:50]])
+    plan.close()
