@@ -11,42 +11,79 @@
 // its edge with a 6-step shuffle search.  Every lane is busy regardless of
 // degree (mean d+ ~ 16 would idle 3/4 of a wave64 in the per-edge form), and
 // hub edges are split across waves by construction.
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
 #include "tables.hpp"
 
+// Edge schedule.  Every oriented edge becomes (s, g): s the endpoint with the
+// smaller out-list (its keys are probed), g the other (its table is probed);
+// ties keep the CSR source as s (TC.cu:146-155).  Edges are ordered by
+// (g, s), so all probes into one table run back to back and hit in L2, and W
+// is the exclusive prefix of the per-edge probe counts d+(s).
 struct rs_tc_plan {
   const rs_tables* t = nullptr;
   int64_t E = 0;
   int64_t total = 0;          // W[E]
   rs::DevBuf W;               // int64[E+1]
+  rs::DevBuf es, eg;          // int32[E]: probing / probed endpoint
   rs::DevBuf counter;         // work-stealing counter of the plain form
+  int64_t E_light = 0;        // edges [0, E_light) have < kHeavy probes
+  int64_t W_light = 0;        // W[E_light]
 };
 
 namespace rs {
 
 constexpr int kTcBlock = 256;
 constexpr int64_t kTcTile = 4096;
+// Edges with at least this many probes are "heavy": one wave covers them 64
+// keys at a time with the edge's metadata in scalar registers.  Lighter edges
+// are flattened 64 edges at a time.
+constexpr int32_t kHeavy = 64;
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
 __global__ void set_i32(int32_t* p, int32_t v) { *p = v; }
 
-// w[e] = min(d+(u), d+(v)); an endpoint outside [0, n) is malformed input
-// (flagged, w = 0), so no later kernel indexes the offsets with it.
-__global__ void tc_work_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
-                               const int32_t* __restrict__ off, int64_t n, int64_t E,
-                               int32_t* __restrict__ w, int32_t* status) {
+// key[e] = g << 32 | s for CSR edge e = (rows[e], cols[e]); an endpoint
+// outside [0, n) is malformed input (flagged, and the edge becomes (0, 0)
+// with no work), so no later kernel indexes the offsets with it.
+__global__ void tc_edge_keys_kernel(const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+                                    const int32_t* __restrict__ off, int64_t n, int64_t E,
+                                    uint64_t* __restrict__ key, int32_t* status) {
   bool bad = false;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t u = rows[e], v = cols[e];
     if (v < 0 || v >= n) {
       bad = true;
-      w[e] = 0;
+      key[e] = ~0ull;
       continue;
     }
     const int32_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
-    w[e] = du < dv ? du : dv;
+    const uint32_t sm = du > dv ? v : u, lg = du > dv ? u : v;
+    const uint64_t heavy = (du < dv ? du : dv) >= kHeavy;
+    key[e] = (heavy << 63) | ((uint64_t)lg << 32) | sm;
   }
   if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
+}
+
+__global__ void tc_edge_split_kernel(const uint64_t* __restrict__ key, const int32_t* __restrict__ off,
+                                     int64_t n, int64_t E, int32_t* __restrict__ es,
+                                     int32_t* __restrict__ eg, int32_t* __restrict__ w) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = key[e];
+    if (k == ~0ull) {
+      es[e] = 0;
+      eg[e] = 0;
+      w[e] = 0;
+      continue;
+    }
+    const int32_t sm = (int32_t)(k & 0xFFFFFFFFull), lg = (int32_t)((k >> 32) & 0x7FFFFFFFull);
+    es[e] = sm;
+    eg[e] = lg;
+    w[e] = off[sm + 1] - off[sm];
+  }
 }
 
 // First-slot fast path for UNROLL independent keys, then the bounded walk
@@ -95,21 +132,31 @@ __device__ __forceinline__ void probe_batch(const int32_t* __restrict__ T, int64
   }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b on XCD b % 8), and
+// each XCD has its own L2, so XCD x takes the contiguous tile range
+// [x*R, (x+1)*R): neighbouring tiles, which probe the same tables, share an L2.
+constexpr int kXcds = 8;
+
 template <int LAYOUT, int UNROLL>
 __global__ void __launch_bounds__(kTcBlock) tc_tiled_kernel(
-    const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+    const int32_t* __restrict__ es, const int32_t* __restrict__ eg,
     const int32_t* __restrict__ off, const int32_t* __restrict__ keys,
     const int32_t* __restrict__ Ls, const int64_t* __restrict__ toff,
     const int32_t* __restrict__ T, int64_t BN, int32_t b, uint32_t M,
     const int64_t* __restrict__ W, int64_t E, int64_t total, int64_t tile,
     unsigned long long* __restrict__ out, int32_t* __restrict__ status) {
   const int lane = __lane_id();
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t local_wave = (int64_t)(blockIdx.x / kXcds) * (kTcBlock / kWave) + wid;
+  const int64_t xcd_waves = (int64_t)(gridDim.x / kXcds) * (kTcBlock / kWave);
   const int64_t ntiles = (total + tile - 1) / tile;
+  const int64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  const int64_t t_begin = xcd * per_xcd;
+  const int64_t t_end = t_begin + per_xcd < ntiles ? t_begin + per_xcd : ntiles;
   uint32_t found = 0;
   bool bad = false;
-  for (int64_t ti = wave; ti < ntiles; ti += nwaves) {
+  for (int64_t ti = t_begin + local_wave; ti < t_end; ti += xcd_waves) {
     const int64_t ts = ti * tile;
     const int64_t te = ts + tile < total ? ts + tile : total;
     // largest e with W[e] <= ts (that edge has work, since ts < total)
@@ -134,13 +181,7 @@ __global__ void __launch_bounds__(kTcBlock) tc_tiled_kernel(
           const int64_t en = we < te ? we : te;
           cnt = (int32_t)(en - s);
           if (cnt > 0) {
-            int32_t u = rows[ei], v = cols[ei];
-            const int32_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
-            if (du > dv) {
-              const int32_t x = u;
-              u = v;
-              v = x;
-            }
+            const int32_t u = es[ei], v = eg[ei];
             kbase = off[u] + (int32_t)(s - ws);
             tb = toff[v];
             Lv = (uint32_t)Ls[v];
@@ -184,6 +225,109 @@ __global__ void __launch_bounds__(kTcBlock) tc_tiled_kernel(
   }
   const uint32_t s = wave_sum(found);
   if (lane == 0 && s) atomicAdd(out, (unsigned long long)s);
+  if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
+}
+
+// Slot-plane probe of U keys of one lane against one table, in lockstep:
+// each step reads the current slot of every pending key (TC.cu:11-31 order:
+// slots 0..b-1 of the home bucket, then the next bucket), so a wave pays the
+// longest chain among its keys rather than the sum.
+template <int LAYOUT, int U>
+__device__ __forceinline__ uint32_t probe_lockstep(const int32_t* __restrict__ Tt, int64_t BN,
+                                                   uint32_t L, int32_t b, uint32_t M,
+                                                   const int32_t (&x)[U], const bool (&act)[U],
+                                                   bool& bad) {
+  uint32_t bucket[U];
+  int32_t k[U];
+  bool pend[U];
+  uint32_t hits = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    bucket[u] = home_bucket<LAYOUT>((uint32_t)x[u], L, M);
+    k[u] = 0;
+    pend[u] = act[u];
+  }
+  const uint32_t bound = L * (uint32_t)b;
+  for (uint32_t step = 0; step < bound; ++step) {
+    int32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = Tt[bucket[u] + (int64_t)k[u] * BN];
+    bool left = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool hit = pend[u] && v[u] == x[u];
+      const bool stop = hit || v[u] == kEmpty;
+      hits += hit;
+      pend[u] = pend[u] && !stop;
+      const int32_t nk = k[u] + 1;
+      const bool wrap = nk == b;
+      k[u] = pend[u] ? (wrap ? 0 : nk) : k[u];
+      bucket[u] = pend[u] && wrap ? ((bucket[u] + 1u) & (L - 1u)) : bucket[u];
+      left |= pend[u];
+    }
+    if (!__any(left)) return hits;
+  }
+  bad = true;
+  return hits;
+}
+
+// Heavy edges (>= kHeavy probes each), ordered by probed table.  Tiles of the
+// probe space [w_lo, w_hi) are dealt to XCDs in contiguous ranges; inside a
+// tile a wave walks its edges one at a time with the edge's endpoints,
+// offsets and table geometry in scalar registers, lanes taking 64*U
+// consecutive keys of the probing list per step.
+template <int LAYOUT, int U>
+__global__ void __launch_bounds__(kTcBlock) tc_heavy_kernel(
+    const int32_t* __restrict__ es, const int32_t* __restrict__ eg,
+    const int32_t* __restrict__ off, const int32_t* __restrict__ keys,
+    const int32_t* __restrict__ Ls, const int64_t* __restrict__ toff,
+    const int32_t* __restrict__ T, int64_t BN, int32_t b, uint32_t M,
+    const int64_t* __restrict__ W, int64_t e_lo, int64_t e_hi, int64_t w_lo, int64_t w_hi,
+    int64_t tile, unsigned long long* __restrict__ out, int32_t* __restrict__ status) {
+  const int lane = __lane_id();
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int xcd = blockIdx.x % kXcds;
+  const int64_t local_wave = (int64_t)(blockIdx.x / kXcds) * (kTcBlock / kWave) + wid;
+  const int64_t xcd_waves = (int64_t)(gridDim.x / kXcds) * (kTcBlock / kWave);
+  const int64_t ntiles = (w_hi - w_lo + tile - 1) / tile;
+  const int64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  const int64_t t_begin = xcd * per_xcd;
+  const int64_t t_end = t_begin + per_xcd < ntiles ? t_begin + per_xcd : ntiles;
+  uint32_t found = 0;
+  bool bad = false;
+  for (int64_t ti = t_begin + local_wave; ti < t_end; ti += xcd_waves) {
+    const int64_t ts = w_lo + ti * tile;
+    const int64_t te = ts + tile < w_hi ? ts + tile : w_hi;
+    int64_t lo = e_lo, hi = e_hi;  // largest e with W[e] <= ts
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (W[mid] <= ts) lo = mid; else hi = mid;
+    }
+    for (int64_t e = lo; e < e_hi; ++e) {
+      const int64_t ws = W[e];
+      if (ws >= te) break;
+      const int64_t we = W[e + 1];
+      const int32_t sv = es[e], gv = eg[e];
+      const int64_t kb = off[sv];
+      const int32_t* Tt = T + toff[gv];
+      const uint32_t L = (uint32_t)Ls[gv];
+      const int32_t j0 = (int32_t)((ws > ts ? ws : ts) - ws);
+      const int32_t j1 = (int32_t)((we < te ? we : te) - ws);
+      for (int32_t j = j0; j < j1; j += kWave * U) {
+        int32_t x[U];
+        bool act[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t jj = j + u * kWave + lane;
+          act[u] = jj < j1;
+          x[u] = act[u] ? keys[kb + jj] : kEmpty;
+        }
+        found += probe_lockstep<LAYOUT, U>(Tt, BN, L, b, M, x, act, bad);
+      }
+    }
+  }
+  const uint32_t sum = wave_sum(found);
+  if (lane == 0 && sum) atomicAdd(out, (unsigned long long)sum);
   if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
 }
 
@@ -234,6 +378,16 @@ __global__ void __launch_bounds__(kTcBlock) tc_plain_kernel(
   if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
 }
 
+__global__ void count_light_kernel(const uint64_t* __restrict__ key, int64_t E,
+                                   unsigned long long* cnt) {
+  uint32_t c = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x)
+    c += (key[e] >> 63) == 0;
+  c = wave_sum(c);
+  if (__lane_id() == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+}
+
 static rs_status check_args(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
                             uint64_t* d_count, const int32_t** T) {
   RS_REQUIRE(t && p && d_count, "tc: null argument");
@@ -248,48 +402,73 @@ using namespace rs;
 
 extern "C" {
 
+static rs_status tc_plan_build(const rs_tables* t, hipStream_t st, rs_tc_plan* p) {
+  RS_TRY(p->W.alloc(sizeof(int64_t) * (size_t)(t->E + 1)));
+  RS_TRY(p->counter.alloc(16));
+  RS_TRY(p->es.alloc(sizeof(int32_t) * (size_t)(t->E > 0 ? t->E : 1)));
+  RS_TRY(p->eg.alloc(sizeof(int32_t) * (size_t)(t->E > 0 ? t->E : 1)));
+  if (t->E == 0) {
+    RS_HIP(hipMemsetAsync(p->W.p, 0, sizeof(int64_t), st));
+    RS_HIP(hipStreamSynchronize(st));
+    return RS_OK;
+  }
+  DevBuf keys, sorted, tmp, w;
+  RS_TRY(keys.alloc(sizeof(uint64_t) * (size_t)t->E));
+  RS_TRY(sorted.alloc(sizeof(uint64_t) * (size_t)t->E));
+  const int g = grid_for(t->E, 256, 16);
+  tc_edge_keys_kernel<<<g, 256, 0, st>>>(t->rows.as<int32_t>(), t->d_elems, t->d_off, t->n, t->E,
+                                        keys.as<uint64_t>(), t->d_status());
+  size_t bytes = 0;
+  if (rocprim::radix_sort_keys(nullptr, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                               (size_t)t->E, 0, 64, st) != hipSuccess) {
+    set_error("rs_tc_plan_create: radix sort sizing failed");
+    return RS_EDEVICE;
+  }
+  RS_TRY(tmp.alloc(bytes));
+  if (rocprim::radix_sort_keys(tmp.p, bytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                               (size_t)t->E, 0, 64, st) != hipSuccess) {
+    set_error("rs_tc_plan_create: radix sort failed");
+    return RS_EDEVICE;
+  }
+  keys.release();
+  RS_TRY(w.alloc(sizeof(int32_t) * (size_t)t->E));
+  tc_edge_split_kernel<<<g, 256, 0, st>>>(sorted.as<uint64_t>(), t->d_off, t->n, t->E,
+                                         p->es.as<int32_t>(), p->eg.as<int32_t>(), w.as<int32_t>());
+  RS_TRY(exclusive_scan_i32_to_i64(w.as<int32_t>(), t->E, p->W.as<int64_t>(), tmp, st));
+  RS_HIP(hipMemcpyAsync(&p->total, p->W.as<int64_t>() + t->E, sizeof(int64_t),
+                        hipMemcpyDeviceToHost, st));
+  // light edges sort first (heavy flag in the key's top bit): count them
+  {
+    DevBuf cnt;
+    RS_TRY(cnt.alloc(16));
+    RS_HIP(hipMemsetAsync(cnt.p, 0, 16, st));
+    count_light_kernel<<<g, 256, 0, st>>>(sorted.as<uint64_t>(), t->E, cnt.as<unsigned long long>());
+    unsigned long long hl = 0;
+    RS_HIP(hipMemcpyAsync(&hl, cnt.p, sizeof(hl), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    p->E_light = (int64_t)hl;
+    RS_HIP(hipMemcpyAsync(&p->W_light, p->W.as<int64_t>() + p->E_light, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, st));
+  }
+  RS_HIP(hipStreamSynchronize(st));
+  rs_status s = rs_tables_status(const_cast<rs_tables*>(t), reinterpret_cast<rs_stream>(st));
+  if (s != RS_OK) {
+    set_error("rs_tc_plan_create: an edge endpoint is outside [0, num_sets)");
+    return RS_EINVAL;
+  }
+  return RS_OK;
+}
+
 rs_status rs_tc_plan_create(const rs_tables* t, rs_stream stream, rs_tc_plan** out) {
   RS_REQUIRE(t && out, "rs_tc_plan_create: null argument");
   *out = nullptr;
-  hipStream_t st = as_stream(stream);
   auto* p = new rs_tc_plan();
   p->t = t;
   p->E = t->E;
-  rs_status s;
-  if ((s = p->W.alloc(sizeof(int64_t) * (size_t)(t->E + 1))) != RS_OK ||
-      (s = p->counter.alloc(16)) != RS_OK) {
+  rs_status s = tc_plan_build(t, as_stream(stream), p);
+  if (s != RS_OK) {
     delete p;
     return s;
-  }
-  if (t->E > 0) {
-    DevBuf w, tmp;
-    if ((s = w.alloc(sizeof(int32_t) * (size_t)t->E)) != RS_OK) {
-      delete p;
-      return s;
-    }
-    tc_work_kernel<<<grid_for(t->E, 256, 16), 256, 0, st>>>(t->rows.as<int32_t>(), t->d_elems,
-                                                            t->d_off, t->n, t->E, w.as<int32_t>(),
-                                                            t->d_status());
-    if ((s = exclusive_scan_i32_to_i64(w.as<int32_t>(), t->E, p->W.as<int64_t>(), tmp, st)) !=
-        RS_OK) {
-      delete p;
-      return s;
-    }
-    if (hipMemcpyAsync(&p->total, p->W.as<int64_t>() + t->E, sizeof(int64_t),
-                       hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-      delete p;
-      set_error("rs_tc_plan_create: device error");
-      return RS_EDEVICE;
-    }
-    if ((s = rs_tables_status(const_cast<rs_tables*>(t), stream)) != RS_OK) {
-      delete p;
-      set_error("rs_tc_plan_create: an edge endpoint is outside [0, num_sets)");
-      return RS_EINVAL;
-    }
-  } else {
-    (void)hipMemsetAsync(p->W.p, 0, sizeof(int64_t), st);
-    (void)hipStreamSynchronize(st);
   }
   *out = p;
   return RS_OK;
@@ -311,20 +490,36 @@ rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t lay
   hipStream_t st = as_stream(stream);
   auto* out = reinterpret_cast<unsigned long long*>(d_count);
   zero_u64<<<1, 1, 0, st>>>(out);
-  if (p->total > 0) {
-    const int32_t* keys = d_probe_keys ? d_probe_keys : t->d_elems;
-    const int64_t ntiles = (p->total + kTcTile - 1) / kTcTile;
-    const int grid = grid_for(ntiles * kWave, kTcBlock, 8);
+  const int32_t* keys = d_probe_keys ? d_probe_keys : t->d_elems;
+  if (p->W_light > 0) {
+    const int64_t ntiles = (p->W_light + kTcTile - 1) / kTcTile;
+    int grid = grid_for(ntiles * kWave, kTcBlock, 8);
+    grid = (grid + kXcds - 1) / kXcds * kXcds;
     if (layout == RS_HIERARCHICAL)
       tc_tiled_kernel<RS_HIERARCHICAL, 4><<<grid, kTcBlock, 0, st>>>(
-          t->rows.as<int32_t>(), t->d_elems, t->d_off, keys, t->lengths.as<int32_t>(),
-          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E,
-          p->total, kTcTile, out, t->d_status());
+          p->es.as<int32_t>(), p->eg.as<int32_t>(), t->d_off, keys, t->lengths.as<int32_t>(),
+          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E_light,
+          p->W_light, kTcTile, out, t->d_status());
     else
       tc_tiled_kernel<RS_NORMAL, 4><<<grid, kTcBlock, 0, st>>>(
-          t->rows.as<int32_t>(), t->d_elems, t->d_off, keys, t->lengths.as<int32_t>(),
-          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E,
-          p->total, kTcTile, out, t->d_status());
+          p->es.as<int32_t>(), p->eg.as<int32_t>(), t->d_off, keys, t->lengths.as<int32_t>(),
+          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E_light,
+          p->W_light, kTcTile, out, t->d_status());
+  }
+  if (p->total > p->W_light) {
+    const int64_t ntiles = (p->total - p->W_light + kTcTile - 1) / kTcTile;
+    int grid = grid_for(ntiles * kWave, kTcBlock, 8);
+    grid = (grid + kXcds - 1) / kXcds * kXcds;
+    if (layout == RS_HIERARCHICAL)
+      tc_heavy_kernel<RS_HIERARCHICAL, 2><<<grid, kTcBlock, 0, st>>>(
+          p->es.as<int32_t>(), p->eg.as<int32_t>(), t->d_off, keys, t->lengths.as<int32_t>(),
+          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E_light,
+          p->E, p->W_light, p->total, kTcTile, out, t->d_status());
+    else
+      tc_heavy_kernel<RS_NORMAL, 2><<<grid, kTcBlock, 0, st>>>(
+          p->es.as<int32_t>(), p->eg.as<int32_t>(), t->d_off, keys, t->lengths.as<int32_t>(),
+          t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E_light,
+          p->E, p->W_light, p->total, kTcTile, out, t->d_status());
   }
   RS_HIP(hipGetLastError());
   return RS_OK;
