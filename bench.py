@@ -188,6 +188,7 @@ def bench_tc(args):
     t = api.HashTables(off, cols, 0.2, 5, 3)
     keys = t.probe_order(1)
     probes, edges = t.tc_work()
+    staged, units, lmax = t.tc_staged()
     _sync_all()
     prep_s = time.time() - t0
     out_t = torch.zeros(1, dtype=torch.int64, device="cuda")
@@ -228,12 +229,13 @@ def bench_tc(args):
         "config": {"workload": f"triangle counting, R-MAT scale {args.scale} edge factor 16, degree-oriented int32 CSR",
                    "scale": args.scale, "vertices": int(off.numel() - 1), "oriented_edges": edges,
                    "probes": probes, "parallelism": "single GPU"},
+        "lds_staged": {"probes": staged, "share": staged / max(probes, 1), "units": units, "max_table_len": lmax},
         "triangles": tri,
         "parity": {"layouts_agree": len(counts) == 1},
         "pairs_per_s": edges / (ms / 1000.0),
         "variants_ms": {k: v[0] for k, v in res.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tc_tiled_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tc_staged_kernel + tc_tiled_kernel",
                      "kernel_ms": ms, "alg_bytes_per_launch": alg_bytes},
         "prep_s": prep_s,
     }

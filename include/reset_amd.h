@@ -125,6 +125,12 @@ rs_status rs_tc_plan_create(const rs_tables* t, rs_stream stream, rs_tc_plan** o
 void rs_tc_plan_free(rs_tc_plan* p);
 /* Total probes (sum over edges of min(d+(u), d+(v))) and edges of the plan. */
 rs_status rs_tc_plan_work(const rs_tc_plan* p, int64_t* h_probes, int64_t* h_edges);
+/* The plan's LDS-staged share: probes whose table is copied into LDS once per
+ * unit (tables probed >= 4x per slot), the number of units and the largest
+ * staged table length.  Diagnostic; no counterpart in the reference (its
+ * kernel reads every table slot from global memory, TC.cu:11-31). */
+rs_status rs_tc_plan_staged(const rs_tc_plan* p, int64_t* h_staged_probes, int64_t* h_units,
+                            int64_t* h_tables_max_len);
 /* Tiled wave64 count into *d_count (uint64, overwritten). */
 rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
                             const int32_t* d_probe_keys, uint64_t* d_count, rs_stream stream);

@@ -137,6 +137,12 @@ class HashTables:
         check(load().rs_tc_plan_work(self.tc_plan(), C.byref(probes), C.byref(edges)))
         return int(probes.value), int(edges.value)
 
+    def tc_staged(self):
+        """(probes served from LDS-staged tables, units, largest staged length)."""
+        probes, units, lmax = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        check(load().rs_tc_plan_staged(self.tc_plan(), C.byref(probes), C.byref(units), C.byref(lmax)))
+        return int(probes.value), int(units.value), int(lmax.value)
+
     def tc_count_async(self, out, layout=NORMAL, probe_keys=None, plain=False, stream=None):
         fn = load().rs_tc_count_plain_async if plain else load().rs_tc_count_async
         check(fn(self._h, self.tc_plan(stream), int(layout), ptr(probe_keys), ptr(out), stream_handle(stream)))

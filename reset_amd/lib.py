@@ -68,6 +68,7 @@ SIGNATURES = {
     "rs_tc_plan_create": (_i32, [_p, _p, _pp]),
     "rs_tc_plan_free": (None, [_p]),
     "rs_tc_plan_work": (_i32, [_p, _p, _p]),
+    "rs_tc_plan_staged": (_i32, [_p, _p, _p, _p]),
     "rs_tc_count_async": (_i32, [_p, _p, _i32, _p, _p, _p]),
     "rs_tc_count_plain_async": (_i32, [_p, _p, _i32, _p, _p, _p]),
     "rs_tc_count": (_i32, [_p, _i32, _p, _p, _p]),

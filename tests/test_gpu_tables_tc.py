@@ -173,6 +173,43 @@ def test_tc_hub_split_and_empty_rows(api, oracle_lib):
     t.close()
 
 
+def test_tc_staged_and_hbm_paths_both_run(api, oracle_lib):
+    """R-MAT scale 16: part of the probes go to LDS-staged tables, the rest to
+    tables read in HBM; the count is the oracle's either way."""
+    from reset_amd import inputs
+
+    off_d, cols_d = inputs.rmat_tc_csr(16, 16, seed=99)
+    want = oracle_lib.tc_merge(off_d.cpu().numpy(), cols_d.cpu().numpy())
+    t = api.HashTables(off_d, cols_d, 0.2, 5, 3)
+    probes, _ = t.tc_work()
+    staged, units, lmax = t.tc_staged()
+    assert 0 < staged < probes and units > 0 and lmax >= 8
+    assert t.tc_count(0) == want and t.tc_count(1) == want
+    assert t.tc_count(1, probe_keys=t.probe_order(1)) == want
+    t.close()
+
+
+def test_tc_complete_graph_all_staged_classes():
+    """K_2300 oriented by id: out-degrees n-1-u give table lengths up to 4096,
+    so tables of all three LDS classes (L <= 1024, 2048, 4096) are staged
+    next to unstaged small tables; count = C(n, 3) exactly."""
+    from reset_amd import api as A
+
+    n = 2300
+    deg = np.arange(n - 1, -1, -1, dtype=np.int64)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(deg, out=off[1:])
+    cols = np.concatenate([np.arange(u + 1, n, dtype=np.int32) for u in range(n)])
+    t = A.HashTables(_cuda(off.astype(np.int32)), _cuda(cols), 0.2, 5, 3)
+    probes, _ = t.tc_work()
+    staged, units, lmax = t.tc_staged()
+    assert lmax == 4096 and 0 < staged < probes
+    want = math.comb(n, 3)
+    assert t.tc_count(0) == want
+    assert t.tc_count(1) == want
+    t.close()
+
+
 @pytest.mark.parametrize("alpha,b", [(0.5, 2), (0.9, 1), (0.1, 8)])
 def test_tc_hyperparameters_do_not_change_the_count(api, oracle_lib, alpha, b):
     from reset_amd import inputs
