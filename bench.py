@@ -50,14 +50,6 @@ def bench_pairs(args):
     from reset_amd import api, inputs
 
     P = args.pairs
-    if args.copy_probe_mb:
-        # a plain device copy of known size, profiled beside the pair kernel to
-        # calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters
-        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        _sync_all()
-        del src, dst
     t0 = time.time()
     batch = inputs.make_pair_batch(P, seed=SEED)
     _sync_all()
@@ -429,6 +421,14 @@ def main(argv=None):
         print("bench.py needs an MI355X GPU", file=sys.stderr)
         return 2
     lib.load()
+    if args.copy_probe_mb:
+        # a plain device copy of known size, profiled beside the measured kernel
+        # to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     out = {"pairs": bench_pairs, "tc": bench_tc, "sss": bench_sss, "ir": bench_ir}[args.workload](args)
     print(json.dumps(out), flush=True)
     return 0

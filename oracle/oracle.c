@@ -179,9 +179,15 @@ int64_t orc_table_check(const int32_t* off, const int32_t* elems, int64_t n,
     const int32_t* Tt = T + toff[t];
     int64_t w = 0;
     for (int32_t j = 0; j < L[t]; j++)
-      for (int32_t k = 0; k < bucket_size; k++) {
+      for (int32_t k = 0, empty_seen = 0; k < bucket_size; k++) {
         const int32_t v = Tt[j + (int64_t)k * BN];
-        if (v == -1) continue;
+        if (v == -1) {
+          empty_seen = 1;
+          continue;
+        }
+        /* inserts take the first EMPTY slot of their walk and never clear
+           one, so a bucket's occupied slots are a prefix of it */
+        if (empty_seen) bad++;
         if (w < cap) a[w] = v;
         w++;
         if (orc_table_search(Tt, BN, home_bucket(layout, v, L[t], max_len), L[t],

@@ -60,8 +60,10 @@ int orc_table_search(const int32_t* Tt, int64_t BN, uint32_t bucket, int32_t L,
 
 /* Invariant check of a table built by anyone (schedule-dependent placement is
  * allowed): every slot lies in its row's region, every stored element belongs
- * to its row, row multiplicities match, and each stored element is reachable
- * from its home bucket before an EMPTY slot.  Returns the number of violations. */
+ * to its row, row multiplicities match, each stored element is reachable
+ * from its home bucket before an EMPTY slot, and the occupied slots of every
+ * bucket form a prefix of it (the property a bucket-at-a-time probe relies
+ * on; reachability implies it).  Returns the number of violations. */
 int64_t orc_table_check(const int32_t* off, const int32_t* elems, int64_t n,
                         const int32_t* L, const int64_t* toff,
                         int32_t bucket_size, int32_t max_len, int32_t layout,

@@ -96,6 +96,20 @@ def test_table_membership_and_invariants(oracle_lib, layout, alpha, b):
     assert oracle_lib.table_check(off, cols, t, T=bad) > 0
 
 
+def test_table_check_flags_a_hole_in_a_bucket(oracle_lib):
+    """An element moved behind an EMPTY slot of its own bucket breaks the
+    bucket-prefix property (and its reachability): flagged."""
+    off = np.array([0, 2], np.int32)
+    cols = np.array([16, 3], np.int32)  # home buckets 0 and 3 of an 8-bucket table
+    t = oracle_lib.build_tables(off, cols, 0.2, 5, 0)
+    assert oracle_lib.table_check(off, cols, t) == 0
+    bad = t.T.copy()
+    BN = t.bucket_num
+    assert bad[0] == 16 and bad[BN] == -1
+    bad[0], bad[BN] = -1, 16  # slot 0 of bucket 0 emptied, the key moved to slot 1
+    assert oracle_lib.table_check(off, cols, t, T=bad) > 0
+
+
 def test_slot_plane_layout_position(oracle_lib):
     """Two sets, keys with a known home bucket: slot k of bucket j of table t
     sits at T[toff[t] + j + k*BN]."""
