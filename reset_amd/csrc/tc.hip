@@ -413,6 +413,65 @@ __device__ __forceinline__ uint32_t probe_lds(const int32_t* __restrict__ tab, u
   return hits;
 }
 
+// Bucket-at-a-time probe against a table staged bucket-major in LDS (slot k
+// of bucket j at tab[j*B + k]).  A bucket's occupied slots always form a
+// prefix of it (an insert takes the first EMPTY slot of its walk and no slot
+// is ever cleared), so the slot walk of TC.cu:11-31 reduces per bucket to:
+// the key is among the B slots -> found; else the last slot is EMPTY ->
+// absent; else the next bucket.  The B compares fold into one xor/min chain.
+// All U keys take their home bucket together; the rare full bucket is
+// followed in a bounded loop (L buckets in all).
+template <int B>
+__device__ __forceinline__ void lds_bucket(const int32_t* __restrict__ tab, uint32_t j, int32_t x,
+                                           bool& hit, bool& full) {
+  const int32_t* bk = tab + j * (uint32_t)B;
+  uint32_t m = ~0u;
+  int32_t last = 0;
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int32_t v = bk[k];
+    m = min(m, (uint32_t)(v ^ x));
+    last = v;
+  }
+  hit = m == 0u;
+  full = last != kEmpty;
+}
+
+template <int LAYOUT, int U, int B>
+__device__ __forceinline__ uint32_t probe_lds_buckets(const int32_t* __restrict__ tab, uint32_t L,
+                                                      uint32_t M, const int32_t (&x)[U],
+                                                      const bool (&act)[U], bool& bad) {
+  uint32_t j[U];
+  bool pend[U];
+  uint32_t hits = 0;
+  bool any_pend = false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    j[u] = home_bucket<LAYOUT>((uint32_t)x[u], L, M);
+    bool hit, full;
+    lds_bucket<B>(tab, j[u], x[u], hit, full);
+    hits += act[u] && hit;
+    pend[u] = act[u] && !hit && full;
+    any_pend |= pend[u];
+  }
+  if (__any(any_pend)) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      uint32_t jj = j[u];
+      bool p = pend[u];
+      for (uint32_t step = 1; step < L && __any(p); ++step) {
+        jj = (jj + 1u) & (L - 1u);
+        bool hit, full;
+        lds_bucket<B>(tab, jj, x[u], hit, full);
+        hits += p && hit;
+        p = p && !hit && full;
+      }
+      bad |= p;
+    }
+  }
+  return hits;
+}
+
 // Staged form, one workgroup per unit at a time (units of one class, split
 // across XCDs in contiguous ranges so a table's units share an L2).  The
 // workgroup copies the unit's table into LDS with 16-byte loads (lengths are
@@ -425,7 +484,7 @@ __device__ __forceinline__ uint32_t probe_lds(const int32_t* __restrict__ tab, u
 // offset in the wave's LDS marker row, a ballot of "tag here" gives the
 // start mask, and the edge of lane l is (edges started before the chunk) +
 // (starts at or below l).  Tags grow per chunk, so the row is never cleared.
-template <int LAYOUT, int NW, int U>
+template <int LAYOUT, int NW, int U, int B>
 __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
     const int32_t* __restrict__ es, const int32_t* __restrict__ off,
     const int32_t* __restrict__ keys, const int32_t* __restrict__ Ls,
@@ -457,7 +516,14 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
     for (uint32_t i = threadIdx.x; i < n4; i += NW * kWave) {
       const uint32_t k = i >> qs, jj = (i & (q - 1u)) << 2;
       const int4 v = *reinterpret_cast<const int4*>(Tt + (int64_t)k * BN + jj);
-      *reinterpret_cast<int4*>(tab + k * L + jj) = v;
+      if constexpr (B == 0) {  // slot planes kept
+        *reinterpret_cast<int4*>(tab + k * L + jj) = v;
+      } else {  // bucket-major
+        tab[(jj + 0) * B + k] = v.x;
+        tab[(jj + 1) * B + k] = v.y;
+        tab[(jj + 2) * B + k] = v.z;
+        tab[(jj + 3) * B + k] = v.w;
+      }
     }
     __syncthreads();
     const int64_t t0 = u_tile[u], t1 = u_tile[u + 1], wb = u_wb[u], we = u_we[u];
@@ -501,9 +567,12 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
             const int32_t p = it + uu * kWave + lane;
             act[uu] = p < S;
             const int32_t d = __shfl(delta, own & (kWave - 1), kWave);
-            x[uu] = act[uu] ? keys[p + d] : kEmpty;
+            x[uu] = keys[act[uu] ? p + d : 0];
           }
-          found += probe_lds<LAYOUT, U>(tab, L, b, M, x, act, bad);
+          if constexpr (B == 0)
+            found += probe_lds<LAYOUT, U>(tab, L, b, M, x, act, bad);
+          else
+            found += probe_lds_buckets<LAYOUT, U, B>(tab, L, M, x, act, bad);
         }
         if (__popcll(__ballot(valid)) < kWave) break;
         e += kWave;
@@ -795,13 +864,13 @@ rs_status rs_tc_plan_staged(const rs_tc_plan* p, int64_t* h_staged_probes, int64
 
 namespace rs {
 
-template <int LAYOUT, int NW>
+template <int LAYOUT, int NW, int B>
 static rs_status launch_staged(const rs_tables* t, const rs_tc_plan* p, int c, const int32_t* T,
                                const int32_t* keys, unsigned long long* out, hipStream_t st) {
   const int64_t L = p->class_lmax[c];
   const size_t lds = stage_lds_bytes(c, L, t->b);
   const int32_t words = (int32_t)(L * t->b);
-  auto kern = tc_staged_kernel<LAYOUT, NW, kStageU>;
+  auto kern = tc_staged_kernel<LAYOUT, NW, kStageU, B>;
   RS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int per_cu = (int)(kLdsBytes / lds);
@@ -821,15 +890,29 @@ static rs_status launch_staged(const rs_tables* t, const rs_tc_plan* p, int c, c
   return RS_OK;
 }
 
+// Bucket sizes with a bucket-at-a-time kernel; others take the slot walk.
+template <int LAYOUT, int NW>
+static rs_status launch_staged_b(const rs_tables* t, const rs_tc_plan* p, int c, const int32_t* T,
+                                 const int32_t* keys, unsigned long long* out, hipStream_t st) {
+  switch (t->b) {
+    case 1: return launch_staged<LAYOUT, NW, 1>(t, p, c, T, keys, out, st);
+    case 2: return launch_staged<LAYOUT, NW, 2>(t, p, c, T, keys, out, st);
+    case 4: return launch_staged<LAYOUT, NW, 4>(t, p, c, T, keys, out, st);
+    case 5: return launch_staged<LAYOUT, NW, 5>(t, p, c, T, keys, out, st);
+    case 8: return launch_staged<LAYOUT, NW, 8>(t, p, c, T, keys, out, st);
+    default: return launch_staged<LAYOUT, NW, 0>(t, p, c, T, keys, out, st);
+  }
+}
+
 template <int LAYOUT>
 static rs_status tc_count_layout(const rs_tables* t, const rs_tc_plan* p, const int32_t* T,
                                  const int32_t* keys, unsigned long long* out, hipStream_t st) {
   if (p->class_units[1] > p->class_units[0])
-    RS_TRY((launch_staged<LAYOUT, kClassWaves[0]>(t, p, 0, T, keys, out, st)));
+    RS_TRY((launch_staged_b<LAYOUT, kClassWaves[0]>(t, p, 0, T, keys, out, st)));
   if (p->class_units[2] > p->class_units[1])
-    RS_TRY((launch_staged<LAYOUT, kClassWaves[1]>(t, p, 1, T, keys, out, st)));
+    RS_TRY((launch_staged_b<LAYOUT, kClassWaves[1]>(t, p, 1, T, keys, out, st)));
   if (p->class_units[3] > p->class_units[2])
-    RS_TRY((launch_staged<LAYOUT, kClassWaves[2]>(t, p, 2, T, keys, out, st)));
+    RS_TRY((launch_staged_b<LAYOUT, kClassWaves[2]>(t, p, 2, T, keys, out, st)));
   const int64_t w_lo = p->wb[3], w_hi = p->wb[4];
   if (w_hi > w_lo) {
     const int64_t ntiles = (w_hi - w_lo + kTcTile - 1) / kTcTile;

@@ -210,7 +210,7 @@ def test_tc_complete_graph_all_staged_classes():
     t.close()
 
 
-@pytest.mark.parametrize("alpha,b", [(0.5, 2), (0.9, 1), (0.1, 8)])
+@pytest.mark.parametrize("alpha,b", [(0.5, 2), (0.9, 1), (0.1, 8), (0.35, 3), (0.3, 4), (0.2, 7)])
 def test_tc_hyperparameters_do_not_change_the_count(api, oracle_lib, alpha, b):
     from reset_amd import inputs
 
