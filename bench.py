@@ -344,7 +344,7 @@ def bench_ir(args):
     # serving at a fixed Poisson arrival rate
     rng = np.random.default_rng(SEED + 7)
     gaps = rng.exponential(1.0 / args.qps, Q)
-    srv = serving.IRServer(t, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch, max_inflight=4)
+    srv = serving.Server(t, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch, max_inflight=4)
     start = time.perf_counter()
     due = start + np.cumsum(gaps)
     for i in range(Q):
@@ -383,8 +383,79 @@ def bench_ir(args):
         "parity": {"served_equals_batch": bool(same), "equals_cpu_merge_sample": bool(merge_ok)},
         "prep_s": prep_s,
     }
+    if args.sss_lookups > 0:
+        out["mixed_serving"] = _mixed_serving(args, t, queries, resh, cnth, roffh)
     t.close()
     return out
+
+
+def _mixed_serving(args, t, queries, resh, cnth, roffh):
+    """IR queries and similarity lookups (J >= 4/5 against a resident Zipf
+    set collection) arriving interleaved on one serving loop."""
+    import numpy as np
+    import torch
+
+    from reset_amd import api, inputs, serving
+
+    soff, scols = inputs.zipf_set_collection(args.sss_sets, 1_000_000, seed=SEED + 8)
+    sidx = api.SSSIndex(soff, scols, 4, 5)
+    soff_h, scols_h = soff.cpu().numpy(), scols.cpu().numpy()
+    rng = np.random.default_rng(SEED + 9)
+    n_s = args.sss_lookups
+    lookups = []
+    for j, i in enumerate(rng.choice(len(soff_h) - 1, n_s).tolist()):
+        r = scols_h[soff_h[i]:soff_h[i + 1]].copy()
+        if j % 2 and len(r) > 5:
+            r[int(rng.integers(0, len(r)))] = 3_000_000 + j  # a token outside the collection
+        lookups.append(np.unique(r).astype(np.int32))
+    lq = np.zeros(n_s + 1, np.int64)
+    np.cumsum([len(r) for r in lookups], out=lq[1:])
+    got = sidx.lookup(torch.from_numpy(lq.astype(np.int32)).cuda(), torch.from_numpy(np.concatenate(lookups)).cuda())
+    keys = got["pairs"].cpu().numpy()
+    batch_sss = [[] for _ in range(n_s)]
+    for k in keys.tolist():
+        batch_sss[k >> 32].append(k & 0xFFFFFFFF)
+    n_ir = min(len(queries), args.queries)
+    rate = args.qps + args.sss_qps
+    gaps = rng.exponential(1.0 / rate, n_ir + n_s)
+    kinds = rng.permutation(np.r_[np.zeros(n_ir, np.int8), np.ones(n_s, np.int8)])
+    srv = serving.Server(t, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch, max_inflight=4,
+                         sss_index=sidx)
+    start = time.perf_counter()
+    due = start + np.cumsum(gaps)
+    ii = si = 0
+    for k in range(n_ir + n_s):
+        d = due[k] - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        if kinds[k] == 0:
+            srv.submit_ir(ii, queries[ii])
+            ii += 1
+        else:
+            srv.submit_sss(1 << 40 | si, lookups[si])
+            si += 1
+    srv.drain()
+    comps = srv.poll_all()
+    lat = {0: [], 1: []}
+    ok_ir = ok_s = True
+    for c in comps:
+        lat[c.kind].append(c.done_ms - c.submit_ms)
+        r = srv.result(c.query_id)
+        if c.kind == 1:
+            ok_s &= r == batch_sss[c.query_id & ((1 << 40) - 1)]
+        else:
+            i = c.query_id
+            ok_ir &= r == resh[roffh[i]:roffh[i] + cnth[i]].tolist()
+    stats = srv.stats()
+    srv.close()
+    sidx.close()
+    pct = lambda v, p: float(np.percentile(np.array(v), p)) if v else None
+    return {"ir_queries": n_ir, "sss_lookups": n_s, "arrival_qps": rate,
+            "sss_collection": {"sets": int(len(soff_h) - 1), "threshold": "4/5"},
+            "ir_latency_ms": {"p50": pct(lat[0], 50), "p99": pct(lat[0], 99)},
+            "sss_latency_ms": {"p50": pct(lat[1], 50), "p99": pct(lat[1], 99)},
+            "sss_matches": int(len(keys)), "server": stats,
+            "parity": {"ir_served_equals_batch": bool(ok_ir), "sss_served_equals_batch": bool(ok_s)}}
 
 
 def main(argv=None):
@@ -398,6 +469,10 @@ def main(argv=None):
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--ir-budget-mb", type=int, default=4096)
     ap.add_argument("--ir-batch", type=int, default=256)
+    ap.add_argument("--sss-lookups", type=int, default=0,
+                    help="ir workload: also serve this many similarity lookups mixed with the queries")
+    ap.add_argument("--sss-qps", type=float, default=500.0)
+    ap.add_argument("--sss-sets", type=int, default=200_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--cpu-sample", type=int, default=50_000,

@@ -178,6 +178,24 @@ rs_status rs_sss_result_work(const rs_sss_result* r, int64_t* h_verify_probes,
 rs_status rs_sss_result_timings(const rs_sss_result* r, float* h_ms3);
 void rs_sss_result_free(rs_sss_result* r);
 
+/* Similarity lookups against a resident collection (the lookup side of the
+ * serving loop; the reference has no lookup form -- its sss.cu verifies a
+ * given candidate list).  The index keeps the collection's global token order
+ * (ascending frequency, ties by id) and its sorted prefix entries for the
+ * threshold p/q; it borrows the collection CSR (ascending, distinct tokens,
+ * ids < 2^22, < 2^21 sets).  A lookup batch gives Q query sets as a CSR
+ * (ascending, distinct tokens; tokens outside the collection are allowed and
+ * rank before all others) and returns every (query, set) with
+ * J >= p/q exactly, as int64 keys (query << 32 | set) in ascending order,
+ * held by an rs_sss_result (rs_sss_result_info / _work / _timings / _free). */
+typedef struct rs_sss_index rs_sss_index;
+rs_status rs_sss_index_build(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                             int32_t p, int32_t q, rs_stream stream, rs_sss_index** out);
+void rs_sss_index_free(rs_sss_index* idx);
+rs_status rs_sss_lookup(const rs_sss_index* idx, const int32_t* d_query_offsets,
+                        int64_t num_queries, const int32_t* d_query_tokens, rs_stream stream,
+                        rs_sss_result** out);
+
 /* ----------------------------------------------------------------------- a11 */
 /* Conjunctive queries.  Replaces ir_kernel (information-retrieval/ir.cu:94-176)
  * and IRBaseline::run_* (baseline/ir/ir.cuh:39-46).  Tables `t` are built over
@@ -194,40 +212,60 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
                             int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
                             rs_stream stream);
 
-/* IR serving loop with continuous batching (no counterpart in the reference,
- * which runs one static batch).  The index and its tables stay resident; a
- * host thread admits submitted queries while their result memory fits
- * `memory_budget_bytes` and at most `max_batch` queries ride in one launch,
- * keeps up to `max_inflight` launches overlapping on their own streams, and
- * completes queries as their launch finishes. */
-typedef struct rs_ir_server rs_ir_server;
-typedef struct rs_ir_completion {
+/* Serving loop with continuous batching (no counterpart in the reference,
+ * which runs one static batch) for two request kinds: IR conjunctive queries
+ * against the resident index `t`, and -- once an rs_sss_index is attached --
+ * set-similarity lookups against its resident collection.  A host thread
+ * admits submitted IR queries while their result memory fits
+ * `memory_budget_bytes`, at most `max_batch` per launch, keeping up to
+ * `max_inflight` launches overlapping on their own streams; a second thread
+ * batches pending lookups (same budget and batch limit) on its own stream,
+ * so both kinds overlap on the device.  Each request completes when its
+ * batch finishes; its result does not depend on what it was batched with.
+ * Query ids are the caller's and must be unique across both kinds. */
+typedef struct rs_server rs_server;
+typedef struct rs_completion {
   int64_t query_id;
   int32_t count;          /* result size, or -1 on error                     */
   int32_t status;         /* RS_OK / error                                   */
   double submit_ms;       /* host clock at submit, ms                        */
   double done_ms;         /* host clock when the result was ready, ms        */
-} rs_ir_completion;
-rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
-                              int32_t max_batch, int32_t max_inflight, rs_ir_server** out);
+  int32_t kind;           /* RS_REQ_IR or RS_REQ_SSS                         */
+  int32_t reserved;
+} rs_completion;
+#define RS_REQ_IR 0
+#define RS_REQ_SSS 1
+rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
+                              int32_t max_batch, int32_t max_inflight, rs_server** out);
 /* Enqueue one query (terms on the host, term0 first).  RS_EBUSY if the
  * pending queue is full. */
-rs_status rs_ir_server_submit(rs_ir_server* s, int64_t query_id, const int32_t* h_terms,
+rs_status rs_server_submit_ir(rs_server* s, int64_t query_id, const int32_t* h_terms,
                               int32_t num_terms);
+/* Attach a similarity index (kept borrowed; it must outlive the server) and
+ * start the lookup thread.  Once per server. */
+rs_status rs_server_attach_sss(rs_server* s, const rs_sss_index* idx);
+/* Enqueue one similarity lookup: the query set's tokens on the host,
+ * strictly ascending and non-negative.  Its result is the ascending list of
+ * collection set ids with J >= the index's p/q.  RS_EINVAL without an
+ * attached index, RS_EBUSY if the pending queue is full. */
+rs_status rs_server_submit_sss(rs_server* s, int64_t query_id, const int32_t* h_tokens,
+                               int32_t num_tokens);
 /* Drain up to `cap` completions; *h_n receives how many were written. */
-rs_status rs_ir_server_poll(rs_ir_server* s, rs_ir_completion* h_out, int32_t cap, int32_t* h_n);
-/* Copy a completed query's result (ascending doc ids if the index elements
- * are sorted) to h_docs (cap entries); *h_count = its full size.  The result
+rs_status rs_server_poll(rs_server* s, rs_completion* h_out, int32_t cap, int32_t* h_n);
+/* Copy a completed request's result (IR: doc ids in term0 order, ascending
+ * if the index elements are sorted; lookup: ascending set ids) to h_docs (cap
+ * entries); *h_count = its full size.  The result
  * is released once it was copied whole; call again with a larger buffer when
  * *h_count > cap. */
-rs_status rs_ir_server_result(rs_ir_server* s, int64_t query_id, int32_t* h_docs, int64_t cap,
+rs_status rs_server_result(rs_server* s, int64_t query_id, int32_t* h_docs, int64_t cap,
                               int64_t* h_count);
 /* Block until every submitted query completed. */
-rs_status rs_ir_server_drain(rs_ir_server* s);
-/* Launches, queries and peak admitted bytes so far. */
-rs_status rs_ir_server_stats(const rs_ir_server* s, int64_t* h_launches, int64_t* h_queries,
+rs_status rs_server_drain(rs_server* s);
+/* Launches (IR and lookup batches), completed requests and peak admitted
+ * bytes so far. */
+rs_status rs_server_stats(const rs_server* s, int64_t* h_launches, int64_t* h_queries,
                              int64_t* h_peak_bytes);
-void rs_ir_server_free(rs_ir_server* s);
+void rs_server_free(rs_server* s);
 
 /* ----------------------------------------------------------------------- a12 */
 /* Pairwise set intersection sizes for P pairs (A_i, B_i) given as two CSR
@@ -281,6 +319,13 @@ rs_status rs_pairs_intersect_indexed_async(rs_pairs_plan* plan, const int32_t* d
                                            const int32_t* d_set_elements, const int32_t* d_u,
                                            const int32_t* d_v, int64_t num_pairs, int32_t* d_sizes,
                                            rs_stream stream);
+/* The batched form over pairs drawn from two set families: pair i is
+ * (A_{u[i]}, B_{v[i]}), e.g. lookup queries against a resident collection. */
+rs_status rs_pairs_intersect_cross_async(rs_pairs_plan* plan, const int32_t* d_a_offsets,
+                                         const int32_t* d_a_elements, const int32_t* d_b_offsets,
+                                         const int32_t* d_b_elements, const int32_t* d_u,
+                                         const int32_t* d_v, int64_t num_pairs, int32_t* d_sizes,
+                                         rs_stream stream);
 /* Capture the batched sequence for these buffers into a hipGraph held by the
  * plan, then replay it with rs_pairs_graph_launch. */
 rs_status rs_pairs_graph_capture(rs_pairs_plan* plan, const int64_t* d_a_off, const int32_t* d_a,

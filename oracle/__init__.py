@@ -73,6 +73,8 @@ def _declare(L):
                                       _p, _p, _i64, _i32, _i32, C.c_int, _p, _p]
     L.orc_sss_float_rule.restype = C.c_int
     L.orc_sss_float_rule.argtypes = [_i32, _i32, _i32, _f32]
+    L.orc_sss_lookup_bruteforce.restype = _i64
+    L.orc_sss_lookup_bruteforce.argtypes = [_p, _p, _i64, _p, _p, _i64, _i32, _i32, _p, _i64]
     L.orc_sss_bruteforce.restype = _i64
     L.orc_sss_bruteforce.argtypes = [_p, _p, _i64, _i32, _i32, _p, _i64]
     L.orc_sss_prefix_join.restype = _i64
@@ -252,6 +254,18 @@ def sss_bruteforce(off, cols, p, q, cap=1 << 22):
     n = lib().orc_sss_bruteforce(_ptr(off), _ptr(cols), len(off) - 1, p, q, _ptr(out), cap)
     if n < 0:
         raise OracleError("bruteforce cap too small")
+    return out[:n].copy()
+
+
+def sss_lookup_bruteforce(off, cols, qoff, qtok, p, q, cap=1 << 22):
+    """Every (query, row) with J >= p/q -> keys (a << 32 | j), ascending."""
+    off, cols = _c(off, np.int32), _c(cols, np.int32)
+    qoff, qtok = _c(qoff, np.int32), _c(qtok, np.int32)
+    out = np.empty(cap, np.int64)
+    n = lib().orc_sss_lookup_bruteforce(_ptr(off), _ptr(cols), len(off) - 1, _ptr(qoff), _ptr(qtok),
+                                        len(qoff) - 1, p, q, _ptr(out), cap)
+    if n < 0:
+        raise OracleError("lookup cap too small")
     return out[:n].copy()
 
 

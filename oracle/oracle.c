@@ -468,6 +468,24 @@ int64_t orc_sss_bruteforce(const int32_t* off, const int32_t* cols, int64_t n,
   return w;
 }
 
+int64_t orc_sss_lookup_bruteforce(const int32_t* off, const int32_t* cols, int64_t n,
+                                  const int32_t* qoff, const int32_t* qtok, int64_t nq,
+                                  int32_t p, int32_t q, int64_t* out, int64_t cap) {
+  int64_t w = 0;
+  for (int64_t a = 0; a < nq; a++) {
+    const int64_t sa = qoff[a + 1] - qoff[a];
+    for (int64_t j = 0; j < n; j++) {
+      const int64_t sj = off[j + 1] - off[j];
+      const int64_t r = merge_count(qtok + qoff[a], sa, cols + off[j], sj);
+      if (jaccard_pass(r, sa, sj, p, q)) {
+        if (w >= cap) return -1;
+        out[w++] = (a << 32) | j;
+      }
+    }
+  }
+  return w;
+}
+
 typedef struct {
   int64_t freq;
   int32_t tok;

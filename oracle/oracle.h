@@ -114,6 +114,13 @@ int orc_sss_float_rule(int32_t r, int32_t size_u, int32_t size_v, float threshol
 int64_t orc_sss_bruteforce(const int32_t* off, const int32_t* cols, int64_t n,
                            int32_t p, int32_t q, int64_t* out, int64_t cap);
 
+/* Every (query a, row j) with J(Q_a, S_j) >= p/q, by sorted merge of each
+ * query against every row (tiny inputs only); keys (a<<32|j) ascending;
+ * returns the count or -1 if cap is too small. */
+int64_t orc_sss_lookup_bruteforce(const int32_t* off, const int32_t* cols, int64_t n,
+                                  const int32_t* qoff, const int32_t* qtok, int64_t nq,
+                                  int32_t p, int32_t q, int64_t* out, int64_t cap);
+
 /* Prefix-filter self-join (Chaudhuri et al. 2006 / Bayardo et al. 2007):
  * global order = ascending token frequency (ties by token id), prefix length
  * s - ceil(p*s/q) + 1, size filter, candidate dedup, exact verification by

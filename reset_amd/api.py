@@ -211,14 +211,7 @@ def ir_result_offsets(index_offsets, query, query_offsets):
 
 
 # ---------------------------------------------------------------- set similarity
-def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, stream=None):
-    """Prefix-filter self-join at Jaccard >= p/q with hash verification.
-    -> dict(pairs: int64 tensor of i<<32|j ascending (device), candidates,
-            verify_probes, ms: [order, candidates, verify])"""
-    _need_cuda(offsets, elements)
-    h = C.c_void_p()
-    check(load().rs_sss_selfjoin(ptr(offsets), offsets.numel() - 1, ptr(elements), int(p), int(q),
-                                 float(load_factor), int(bucket_size), stream_handle(stream), C.byref(h)))
+def _sss_result(h, device):
     try:
         npairs, ncand, dptr = C.c_int64(0), C.c_int64(0), C.c_void_p()
         check(load().rs_sss_result_info(h, C.byref(npairs), C.byref(ncand), C.byref(dptr)))
@@ -226,7 +219,7 @@ def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, st
         check(load().rs_sss_result_work(h, C.byref(probes), C.byref(elems)))
         ms = (C.c_float * 3)()
         check(load().rs_sss_result_timings(h, ms))
-        pairs = torch.empty(max(npairs.value, 1), dtype=torch.int64, device=offsets.device)
+        pairs = torch.empty(max(npairs.value, 1), dtype=torch.int64, device=device)
         if npairs.value:
             torch.cuda.synchronize()
             rc = _hip().hipMemcpy(C.c_void_p(pairs.data_ptr()), dptr, C.c_size_t(8 * npairs.value), 3)
@@ -236,6 +229,49 @@ def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, st
                     verify_elements=int(elems.value), ms=[float(x) for x in ms])
     finally:
         load().rs_sss_result_free(h)
+
+
+def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, stream=None):
+    """Prefix-filter self-join at Jaccard >= p/q with hash verification.
+    -> dict(pairs: int64 tensor of i<<32|j ascending (device), candidates,
+            verify_probes, ms: [order, candidates, verify])"""
+    _need_cuda(offsets, elements)
+    h = C.c_void_p()
+    check(load().rs_sss_selfjoin(ptr(offsets), offsets.numel() - 1, ptr(elements), int(p), int(q),
+                                 float(load_factor), int(bucket_size), stream_handle(stream), C.byref(h)))
+    return _sss_result(h, offsets.device)
+
+
+class SSSIndex:
+    """Resident similarity-lookup index over a set collection (borrowed CSR)."""
+
+    def __init__(self, offsets, elements, p=4, q=5, stream=None):
+        _need_cuda(offsets, elements)
+        assert offsets.dtype == torch.int32 and elements.dtype == torch.int32
+        self.offsets, self.elements, self.p, self.q = offsets, elements, int(p), int(q)
+        self._h = C.c_void_p()
+        check(load().rs_sss_index_build(ptr(offsets), offsets.numel() - 1, ptr(elements), self.p, self.q,
+                                        stream_handle(stream), C.byref(self._h)))
+
+    def lookup(self, query_offsets, query_tokens, stream=None):
+        """-> dict(pairs: int64 (query << 32 | set) ascending, candidates, ...)"""
+        _need_cuda(query_offsets, query_tokens)
+        assert query_offsets.dtype == torch.int32 and query_tokens.dtype == torch.int32
+        h = C.c_void_p()
+        check(load().rs_sss_lookup(self._h, ptr(query_offsets), query_offsets.numel() - 1, ptr(query_tokens),
+                                   stream_handle(stream), C.byref(h)))
+        return _sss_result(h, query_offsets.device)
+
+    def close(self):
+        if self._h:
+            load().rs_sss_index_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- pairs

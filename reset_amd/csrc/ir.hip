@@ -252,7 +252,7 @@ struct IrSlot {
   IrScratch sc;
 };
 
-struct rs_ir_server {
+struct rs_server {
   const rs_tables* t = nullptr;
   int32_t layout = 0;
   int device = 0;
@@ -263,12 +263,21 @@ struct rs_ir_server {
   std::mutex mu;
   std::condition_variable cv;
   std::deque<IrPending> pending;
-  std::deque<rs_ir_completion> done;
+  std::deque<rs_completion> done;
   std::unordered_map<int64_t, std::vector<int32_t>> results;
   int64_t submitted = 0, completed = 0, launches = 0, admitted_bytes = 0, peak_bytes = 0;
   bool stop = false;
   rs_status worker_error = RS_OK;
   std::thread worker;
+  // similarity lookups (second request kind), served by their own thread
+  const rs_sss_index* sss = nullptr;
+  std::deque<IrPending> sss_pending;  // terms = the query set's tokens
+  std::thread sss_worker;
+  hipStream_t sss_stream = nullptr;
+  int32_t* h_sss_tok = nullptr;   // pinned staging
+  int32_t* h_sss_off = nullptr;
+  int64_t* h_sss_keys = nullptr;
+  int64_t cap_sss_tok = 0, cap_sss_off = 0, cap_sss_keys = 0;
 };
 
 template <typename T>
@@ -318,7 +327,7 @@ static rs_status slot_reserve(IrSlot& s, int64_t Q, int64_t nterms, int64_t tota
 }
 
 // Launch a batch on a slot: H2D query arrays, query kernels, pack, D2H counts.
-static rs_status slot_launch(rs_ir_server* S, IrSlot& s) {
+static rs_status slot_launch(rs_server* S, IrSlot& s) {
   const int64_t Q = (int64_t)s.batch.size();
   int64_t nterms = 0, total = 0;
   for (auto& p : s.batch) {
@@ -358,7 +367,7 @@ static rs_status slot_launch(rs_ir_server* S, IrSlot& s) {
   return RS_OK;
 }
 
-static void server_loop(rs_ir_server* S) {
+static void server_loop(rs_server* S) {
   (void)hipSetDevice(S->device);
   while (true) {
     bool progressed = false;
@@ -382,7 +391,7 @@ static void server_loop(rs_ir_server* S) {
           const int32_t c = s.h_count[i];
           const int64_t o = s.h_outoff[i];
           S->results[s.batch[i].id] = std::vector<int32_t>(s.h_dense + o, s.h_dense + o + c);
-          S->done.push_back(rs_ir_completion{s.batch[i].id, c, RS_OK, s.batch[i].submit_ms, t});
+          S->done.push_back(rs_completion{s.batch[i].id, c, RS_OK, s.batch[i].submit_ms, t});
         }
         S->completed += (int64_t)s.batch.size();
         S->admitted_bytes -= s.bytes;
@@ -420,7 +429,7 @@ static void server_loop(rs_ir_server* S) {
         lk.lock();
         if (st != RS_OK) {
           const double t = now_ms();
-          for (auto& p : s.batch) S->done.push_back(rs_ir_completion{p.id, -1, st, p.submit_ms, t});
+          for (auto& p : s.batch) S->done.push_back(rs_completion{p.id, -1, st, p.submit_ms, t});
           S->completed += (int64_t)s.batch.size();
           S->admitted_bytes -= s.bytes;
           s.batch.clear();
@@ -441,6 +450,111 @@ static void server_loop(rs_ir_server* S) {
                          [S] { return S->stop || !S->pending.empty(); });
       }
     }
+  }
+}
+
+// Footprint charged to a lookup while it is admitted: candidate keys, pair
+// sizes and staging for up to 64 candidates per query token.
+static int64_t sss_footprint(const IrPending& p) { return 1024 + 64 * 16 * (int64_t)p.terms.size(); }
+
+// One lookup batch: stage the query sets, run the lookup on the server's
+// lookup stream (the pipeline synchronizes that stream between its stages),
+// split the (query << 32 | set) keys into per-request results.
+static rs_status sss_run_batch(rs_server* S, std::vector<IrPending>& batch) {
+  const int64_t Q = (int64_t)batch.size();
+  int64_t ntok = 0;
+  for (auto& p : batch) ntok += (int64_t)p.terms.size();
+  if (Q + 1 > S->cap_sss_off) {
+    RS_TRY(host_alloc(S->h_sss_off, 2 * (Q + 1)));
+    S->cap_sss_off = 2 * (Q + 1);
+  }
+  if ((ntok > 0 ? ntok : 1) > S->cap_sss_tok) {
+    RS_TRY(host_alloc(S->h_sss_tok, 2 * (ntok > 0 ? ntok : 1)));
+    S->cap_sss_tok = 2 * (ntok > 0 ? ntok : 1);
+  }
+  int64_t w = 0;
+  for (int64_t i = 0; i < Q; ++i) {
+    S->h_sss_off[i] = (int32_t)w;
+    for (int32_t x : batch[i].terms) S->h_sss_tok[w++] = x;
+  }
+  S->h_sss_off[Q] = (int32_t)w;
+  hipStream_t st = S->sss_stream;
+  DevBuf d_off, d_tok;
+  RS_TRY(d_off.alloc(sizeof(int32_t) * (size_t)(Q + 1)));
+  RS_TRY(d_tok.alloc(sizeof(int32_t) * (size_t)(ntok > 0 ? ntok : 1)));
+  RS_HIP(hipMemcpyAsync(d_off.p, S->h_sss_off, sizeof(int32_t) * (size_t)(Q + 1), hipMemcpyHostToDevice, st));
+  if (ntok > 0)
+    RS_HIP(hipMemcpyAsync(d_tok.p, S->h_sss_tok, sizeof(int32_t) * (size_t)ntok, hipMemcpyHostToDevice, st));
+  rs_sss_result* R = nullptr;
+  RS_TRY(rs_sss_lookup(S->sss, d_off.as<int32_t>(), Q, d_tok.as<int32_t>(), reinterpret_cast<rs_stream>(st), &R));
+  struct RGuard {
+    rs_sss_result* r;
+    ~RGuard() { rs_sss_result_free(r); }
+  } rg{R};
+  int64_t K = 0;
+  const int64_t* d_keys = nullptr;
+  RS_TRY(rs_sss_result_info(R, &K, nullptr, &d_keys));
+  if (K > S->cap_sss_keys) {
+    RS_TRY(host_alloc(S->h_sss_keys, 2 * K));
+    S->cap_sss_keys = 2 * K;
+  }
+  if (K > 0) {
+    RS_HIP(hipMemcpyAsync(S->h_sss_keys, d_keys, sizeof(int64_t) * (size_t)K, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  const double t = now_ms();
+  std::vector<std::vector<int32_t>> per((size_t)Q);
+  for (int64_t i = 0; i < K; ++i) {
+    const uint64_t k = (uint64_t)S->h_sss_keys[i];
+    per[(size_t)(k >> 32)].push_back((int32_t)(k & 0xFFFFFFFFull));
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (int64_t i = 0; i < Q; ++i) {
+    const int32_t c = (int32_t)per[(size_t)i].size();
+    S->results[batch[(size_t)i].id] = std::move(per[(size_t)i]);
+    S->done.push_back(rs_completion{batch[(size_t)i].id, c, RS_OK, batch[(size_t)i].submit_ms, t,
+                                    RS_REQ_SSS, 0});
+  }
+  return RS_OK;
+}
+
+static void sss_loop(rs_server* S) {
+  (void)hipSetDevice(S->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    S->cv.wait(lk, [S] { return S->stop || !S->sss_pending.empty(); });
+    if (S->stop) break;
+    // admit FIFO while the batch fits the shared memory budget
+    std::vector<IrPending> batch;
+    int64_t bytes = 0;
+    while (!S->sss_pending.empty() && (int32_t)batch.size() < S->max_batch) {
+      const int64_t fb = sss_footprint(S->sss_pending.front());
+      // over budget: wait for completions, unless nothing at all is admitted
+      if (S->admitted_bytes + bytes + fb > S->budget && !(batch.empty() && S->admitted_bytes == 0))
+        break;
+      bytes += fb;
+      batch.push_back(std::move(S->sss_pending.front()));
+      S->sss_pending.pop_front();
+    }
+    if (batch.empty()) {
+      S->cv.wait_for(lk, std::chrono::microseconds(200));
+      continue;
+    }
+    S->admitted_bytes += bytes;
+    S->peak_bytes = S->admitted_bytes > S->peak_bytes ? S->admitted_bytes : S->peak_bytes;
+    S->launches++;
+    lk.unlock();
+    rs_status st = sss_run_batch(S, batch);
+    lk.lock();
+    if (st != RS_OK) {
+      const double t = now_ms();
+      for (auto& p : batch)
+        S->done.push_back(rs_completion{p.id, -1, st, p.submit_ms, t, RS_REQ_SSS, 0});
+      S->worker_error = st;
+    }
+    S->completed += (int64_t)batch.size();
+    S->admitted_bytes -= bytes;
+    S->cv.notify_all();
   }
 }
 
@@ -465,15 +579,15 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
   return RS_OK;
 }
 
-rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
-                              int32_t max_batch, int32_t max_inflight, rs_ir_server** out) {
-  RS_REQUIRE(t && out, "rs_ir_server_create: null argument");
+rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
+                              int32_t max_batch, int32_t max_inflight, rs_server** out) {
+  RS_REQUIRE(t && out, "rs_server_create: null argument");
   RS_REQUIRE(memory_budget_bytes > 0 && max_batch > 0 && max_inflight > 0 && max_inflight <= 16,
-             "rs_ir_server_create: bad limits");
+             "rs_server_create: bad limits");
   const int32_t* T = nullptr;
   RS_TRY(require_layout(t, layout, &T));
   *out = nullptr;
-  auto* S = new rs_ir_server();
+  auto* S = new rs_server();
   S->t = t;
   S->layout = layout;
   S->budget = memory_budget_bytes;
@@ -483,7 +597,7 @@ rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory
   if (hipMemcpy(S->h_idx_off.data(), t->d_off, sizeof(int32_t) * (size_t)(t->n + 1),
                 hipMemcpyDeviceToHost) != hipSuccess) {
     delete S;
-    set_error("rs_ir_server_create: cannot read index offsets");
+    set_error("rs_server_create: cannot read index offsets");
     return RS_EDEVICE;
   }
   for (int i = 0; i < max_inflight; ++i) S->slots.push_back(std::make_unique<IrSlot>());
@@ -493,7 +607,7 @@ rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory
         hipEventCreateWithFlags(&s.ev_counts, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming) != hipSuccess) {
       delete S;
-      set_error("rs_ir_server_create: stream/event creation failed");
+      set_error("rs_server_create: stream/event creation failed");
       return RS_EDEVICE;
     }
   }
@@ -502,20 +616,20 @@ rs_status rs_ir_server_create(const rs_tables* t, int32_t layout, int64_t memory
   return RS_OK;
 }
 
-rs_status rs_ir_server_submit(rs_ir_server* S, int64_t query_id, const int32_t* h_terms,
+rs_status rs_server_submit_ir(rs_server* S, int64_t query_id, const int32_t* h_terms,
                               int32_t num_terms) {
-  RS_REQUIRE(S && (h_terms || num_terms == 0) && num_terms >= 0, "rs_ir_server_submit: bad args");
+  RS_REQUIRE(S && (h_terms || num_terms == 0) && num_terms >= 0, "rs_server_submit: bad args");
   IrPending p;
   p.id = query_id;
   p.submit_ms = now_ms();
   p.terms.assign(h_terms, h_terms + num_terms);
   for (int32_t x : p.terms)
-    RS_REQUIRE(x >= 0 && x < S->t->n, "rs_ir_server_submit: term id %d out of range", x);
+    RS_REQUIRE(x >= 0 && x < S->t->n, "rs_server_submit: term id %d out of range", x);
   p.n0 = num_terms > 0 ? S->h_idx_off[p.terms[0] + 1] - S->h_idx_off[p.terms[0]] : 0;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->pending.size() >= (size_t)1 << 22) {
-      set_error("rs_ir_server_submit: pending queue full");
+      set_error("rs_server_submit: pending queue full");
       return RS_EBUSY;
     }
     S->pending.push_back(std::move(p));
@@ -525,8 +639,53 @@ rs_status rs_ir_server_submit(rs_ir_server* S, int64_t query_id, const int32_t* 
   return RS_OK;
 }
 
-rs_status rs_ir_server_poll(rs_ir_server* S, rs_ir_completion* h_out, int32_t cap, int32_t* h_n) {
-  RS_REQUIRE(S && h_n && (h_out || cap == 0), "rs_ir_server_poll: bad args");
+rs_status rs_server_attach_sss(rs_server* S, const rs_sss_index* idx) {
+  RS_REQUIRE(S && idx, "rs_server_attach_sss: null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  RS_REQUIRE(!S->sss, "rs_server_attach_sss: an index is already attached");
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(S->device);
+  const hipError_t e = hipStreamCreateWithFlags(&S->sss_stream, hipStreamNonBlocking);
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess) {
+    S->sss_stream = nullptr;
+    set_error("rs_server_attach_sss: stream creation failed");
+    return RS_EDEVICE;
+  }
+  S->sss = idx;
+  S->sss_worker = std::thread(sss_loop, S);
+  return RS_OK;
+}
+
+rs_status rs_server_submit_sss(rs_server* S, int64_t query_id, const int32_t* h_tokens,
+                               int32_t num_tokens) {
+  RS_REQUIRE(S && (h_tokens || num_tokens == 0) && num_tokens >= 0, "rs_server_submit_sss: bad args");
+  RS_REQUIRE(num_tokens < (1 << 21), "rs_server_submit_sss: a query has 2^21 or more tokens");
+  IrPending p;
+  p.id = query_id;
+  p.submit_ms = now_ms();
+  p.terms.assign(h_tokens, h_tokens + num_tokens);
+  p.n0 = 0;
+  for (int32_t i = 0; i < num_tokens; ++i)
+    RS_REQUIRE(p.terms[i] >= 0 && (i == 0 || p.terms[i - 1] < p.terms[i]),
+               "rs_server_submit_sss: tokens must be non-negative and strictly ascending");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    RS_REQUIRE(S->sss, "rs_server_submit_sss: no similarity index attached");
+    if (S->sss_pending.size() >= (size_t)1 << 22) {
+      set_error("rs_server_submit_sss: pending queue full");
+      return RS_EBUSY;
+    }
+    S->sss_pending.push_back(std::move(p));
+    S->submitted++;
+  }
+  S->cv.notify_all();
+  return RS_OK;
+}
+
+rs_status rs_server_poll(rs_server* S, rs_completion* h_out, int32_t cap, int32_t* h_n) {
+  RS_REQUIRE(S && h_n && (h_out || cap == 0), "rs_server_poll: bad args");
   std::lock_guard<std::mutex> lk(S->mu);
   int32_t n = 0;
   while (n < cap && !S->done.empty()) {
@@ -537,32 +696,32 @@ rs_status rs_ir_server_poll(rs_ir_server* S, rs_ir_completion* h_out, int32_t ca
   return S->worker_error;
 }
 
-rs_status rs_ir_server_result(rs_ir_server* S, int64_t query_id, int32_t* h_docs, int64_t cap,
+rs_status rs_server_result(rs_server* S, int64_t query_id, int32_t* h_docs, int64_t cap,
                               int64_t* h_count) {
-  RS_REQUIRE(S && h_count, "rs_ir_server_result: bad args");
+  RS_REQUIRE(S && h_count, "rs_server_result: bad args");
   std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->results.find(query_id);
-  RS_REQUIRE(it != S->results.end(), "rs_ir_server_result: query %lld has no result",
+  RS_REQUIRE(it != S->results.end(), "rs_server_result: query %lld has no result",
              (long long)query_id);
   *h_count = (int64_t)it->second.size();
-  RS_REQUIRE(h_docs || it->second.empty() || cap == 0, "rs_ir_server_result: null buffer");
+  RS_REQUIRE(h_docs || it->second.empty() || cap == 0, "rs_server_result: null buffer");
   const int64_t n = *h_count < cap ? *h_count : cap;
   if (n > 0) memcpy(h_docs, it->second.data(), sizeof(int32_t) * (size_t)n);
   if (n == *h_count) S->results.erase(it);  // kept until fully copied out
   return RS_OK;
 }
 
-rs_status rs_ir_server_drain(rs_ir_server* S) {
-  RS_REQUIRE(S, "rs_ir_server_drain: null server");
+rs_status rs_server_drain(rs_server* S) {
+  RS_REQUIRE(S, "rs_server_drain: null server");
   std::unique_lock<std::mutex> lk(S->mu);
   S->cv.wait(lk, [S] { return S->completed >= S->submitted; });
   return S->worker_error;
 }
 
-rs_status rs_ir_server_stats(const rs_ir_server* S, int64_t* h_launches, int64_t* h_queries,
+rs_status rs_server_stats(const rs_server* S, int64_t* h_launches, int64_t* h_queries,
                              int64_t* h_peak_bytes) {
-  RS_REQUIRE(S, "rs_ir_server_stats: null server");
-  auto* M = const_cast<rs_ir_server*>(S);
+  RS_REQUIRE(S, "rs_server_stats: null server");
+  auto* M = const_cast<rs_server*>(S);
   std::lock_guard<std::mutex> lk(M->mu);
   if (h_launches) *h_launches = S->launches;
   if (h_queries) *h_queries = S->completed;
@@ -570,7 +729,7 @@ rs_status rs_ir_server_stats(const rs_ir_server* S, int64_t* h_launches, int64_t
   return RS_OK;
 }
 
-void rs_ir_server_free(rs_ir_server* S) {
+void rs_server_free(rs_server* S) {
   if (!S) return;
   {
     std::lock_guard<std::mutex> lk(S->mu);
@@ -578,6 +737,13 @@ void rs_ir_server_free(rs_ir_server* S) {
   }
   S->cv.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  if (S->sss_worker.joinable()) S->sss_worker.join();
+  if (S->sss_stream) {
+    (void)hipStreamSynchronize(S->sss_stream);
+    (void)hipStreamDestroy(S->sss_stream);
+  }
+  for (void* p : {(void*)S->h_sss_tok, (void*)S->h_sss_off, (void*)S->h_sss_keys})
+    if (p) (void)hipHostFree(p);
   for (auto& sp : S->slots) {
     IrSlot& s = *sp;
     if (s.stream) (void)hipStreamSynchronize(s.stream);

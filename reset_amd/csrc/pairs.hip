@@ -62,23 +62,27 @@ struct PairSrcCsr {
   }
 };
 
+// Pair i = (A_{u[i]}, B_{v[i]}) of two CSR families with int32 offsets (one
+// family twice for a self-join's candidates).
 struct PairSrcIdx {
-  const int32_t* off;
-  const int32_t* el;
+  const int32_t* a_off;
+  const int32_t* a_el;
+  const int32_t* b_off;
+  const int32_t* b_el;
   const int32_t* u;
   const int32_t* v;
   __device__ __forceinline__ void get(int64_t i, const int32_t*& A, int32_t& na, const int32_t*& B,
                                       int32_t& nb) const {
     const int32_t x = u[i], y = v[i];
-    const int32_t a0 = off[x], b0 = off[y];
-    A = el + a0;
-    na = off[x + 1] - a0;
-    B = el + b0;
-    nb = off[y + 1] - b0;
+    const int32_t a0 = a_off[x], b0 = b_off[y];
+    A = a_el + a0;
+    na = a_off[x + 1] - a0;
+    B = b_el + b0;
+    nb = b_off[y + 1] - b0;
   }
   __device__ __forceinline__ int64_t total(int64_t i) const {
     const int32_t x = u[i], y = v[i];
-    return (int64_t)(off[x + 1] - off[x]) + (off[y + 1] - off[y]);
+    return (int64_t)(a_off[x + 1] - a_off[x]) + (b_off[y + 1] - b_off[y]);
   }
 };
 
@@ -746,7 +750,24 @@ rs_status rs_pairs_intersect_indexed_async(rs_pairs_plan* plan, const int32_t* d
   RS_REQUIRE(num_pairs >= 0 && num_pairs <= plan->max_pairs,
              "rs_pairs_intersect_indexed_async: %lld pairs exceed the plan's %lld",
              (long long)num_pairs, (long long)plan->max_pairs);
-  return pairs_enqueue<PairSrcIdx>(plan, PairSrcIdx{d_set_offsets, d_set_elements, d_u, d_v}, num_pairs,
+  return pairs_enqueue<PairSrcIdx>(plan, PairSrcIdx{d_set_offsets, d_set_elements, d_set_offsets,
+                                                    d_set_elements, d_u, d_v}, num_pairs,
+                                   d_sizes, as_stream(stream));
+}
+
+rs_status rs_pairs_intersect_cross_async(rs_pairs_plan* plan, const int32_t* d_a_offsets,
+                                         const int32_t* d_a_elements, const int32_t* d_b_offsets,
+                                         const int32_t* d_b_elements, const int32_t* d_u,
+                                         const int32_t* d_v, int64_t num_pairs, int32_t* d_sizes,
+                                         rs_stream stream) {
+  RS_REQUIRE(plan && d_a_offsets && d_a_elements && d_b_offsets && d_b_elements &&
+                 (num_pairs == 0 || (d_u && d_v && d_sizes)),
+             "rs_pairs_intersect_cross_async: null argument");
+  RS_REQUIRE(num_pairs >= 0 && num_pairs <= plan->max_pairs,
+             "rs_pairs_intersect_cross_async: %lld pairs exceed the plan's %lld",
+             (long long)num_pairs, (long long)plan->max_pairs);
+  return pairs_enqueue<PairSrcIdx>(plan, PairSrcIdx{d_a_offsets, d_a_elements, d_b_offsets,
+                                                    d_b_elements, d_u, d_v}, num_pairs,
                                    d_sizes, as_stream(stream));
 }
 

@@ -20,6 +20,15 @@
 
 #include "sss_verify.hpp"
 
+struct rs_sss_index {
+  const int32_t* off = nullptr;  // borrowed collection CSR
+  const int32_t* el = nullptr;
+  int64_t n = 0, E = 0, V = 0, M = 0;
+  int32_t p = 0, q = 0;
+  rs::DevBuf rank;  // int32[V]: global rank of each token id
+  rs::DevBuf ent;   // uint64[M]: sorted prefix entries rank << 42 | size << 21 | set
+};
+
 struct rs_sss_result {
   rs::DevBuf pairs;  // int64 keys i << 32 | j, ascending
   int64_t num_pairs = 0;
@@ -176,14 +185,15 @@ __global__ void split_pairs_kernel(const uint64_t* __restrict__ cand, int64_t n,
 
 // Exact rule on the pair sizes r: flag iff q*r >= p*(|S_u|+|S_v|-r) > ... ;
 // also accumulates sum(min) and sum(|S_u|+|S_v|) for the work accounting.
-__global__ void jaccard_flags_kernel(const int32_t* __restrict__ off, const int32_t* __restrict__ u,
+__global__ void jaccard_flags_kernel(const int32_t* __restrict__ a_off,
+                                     const int32_t* __restrict__ b_off, const int32_t* __restrict__ u,
                                      const int32_t* __restrict__ v, const int32_t* __restrict__ r,
                                      int64_t n, int32_t p, int32_t q, int32_t* __restrict__ flags,
                                      unsigned long long* work) {
   unsigned long long smin = 0, ssum = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t su = off[u[i] + 1] - off[u[i]], sv = off[v[i] + 1] - off[v[i]];
+    const int64_t su = a_off[u[i] + 1] - a_off[u[i]], sv = b_off[v[i] + 1] - b_off[v[i]];
     const int64_t U = su + sv - r[i];
     flags[i] = U > 0 && (int64_t)q * r[i] >= (int64_t)p * U;
     smin += (unsigned long long)(su < sv ? su : sv);
@@ -229,6 +239,95 @@ static unsigned bits_for(uint64_t max_value) {
   return b;
 }
 
+// Stages 1-2: rank[t] for every token t < V (V = max token + 1) by
+// (frequency, id).  err: device error word (kErrBadInput on a negative id).
+static rs_status token_ranks(const int32_t* d_el, int64_t E, DevBuf& tmp, hipStream_t st,
+                             int32_t* d_scratch, DevBuf& rank, int64_t* V_out) {
+  int32_t* d_max = d_scratch;  // [0] max token, [1] error bits
+  RS_HIP(hipMemsetAsync(d_max, 0xFF, sizeof(int32_t), st));
+  if (E > 0) max_token_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, d_max, d_max + 1);
+  int32_t hm[2];
+  RS_HIP(hipMemcpyAsync(hm, d_max, sizeof(hm), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(!hm[1], "sss: negative token id");
+  const int64_t V = (int64_t)hm[0] + 1;
+  RS_REQUIRE(V < (1ll << 22), "sss: token ids must be < 2^22");
+  DevBuf sorted_tok, freq, okeys, okeys_sorted;
+  RS_TRY(sorted_tok.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  RS_TRY(freq.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
+  RS_HIP(hipMemsetAsync(freq.p, 0, sizeof(int32_t) * (size_t)(V > 0 ? V : 1), st));
+  if (E > 0) {
+    RS_TRY(sort_keys<int32_t>(d_el, sorted_tok.as<int32_t>(), E, bits_for((uint64_t)V), tmp, st));
+    run_freq_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(sorted_tok.as<int32_t>(), E, freq.as<int32_t>());
+  }
+  sorted_tok.release();
+  RS_TRY(okeys.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
+  RS_TRY(okeys_sorted.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
+  RS_TRY(rank.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
+  if (V > 0) {
+    order_keys_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(freq.as<int32_t>(), V, okeys.as<uint64_t>());
+    RS_TRY(sort_keys<uint64_t>(okeys.as<uint64_t>(), okeys_sorted.as<uint64_t>(), V,
+                               32 + bits_for((uint64_t)(E > 0 ? E : 1)), tmp, st));
+    rank_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(okeys_sorted.as<uint64_t>(), V, rank.as<int32_t>());
+  }
+  *V_out = V;
+  return RS_OK;
+}
+
+// Stage 3: each set's tokens as ranks, ascending within the set.
+static rs_status ranked_sets(const int32_t* d_off, int64_t n, const int32_t* d_el, int64_t E,
+                             const DevBuf& rank, int64_t V, DevBuf& tmp, hipStream_t st,
+                             DevBuf& ranked_sorted) {
+  DevBuf ranked;
+  RS_TRY(ranked.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  RS_TRY(ranked_sorted.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
+  if (E > 0) {
+    map_rank_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, rank.as<int32_t>(), ranked.as<int32_t>());
+    size_t bytes = 0;
+    RS_ROC(rocprim::segmented_radix_sort_keys(nullptr, bytes, ranked.as<int32_t>(),
+                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
+                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
+    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+    RS_ROC(rocprim::segmented_radix_sort_keys(tmp.p, bytes, ranked.as<int32_t>(),
+                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
+                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
+  }
+  return RS_OK;
+}
+
+// Stage 4: prefix entries (rank << 42 | size << 21 | set), sorted.
+static rs_status prefix_entries(const int32_t* d_off, int64_t n, const DevBuf& ranked_sorted,
+                                int32_t p, int32_t q, DevBuf& tmp, hipStream_t st,
+                                int32_t* d_scratch, DevBuf& ent_sorted, int64_t* M_out) {
+  DevBuf pl, poff, ent;
+  RS_TRY(pl.alloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
+  RS_TRY(poff.alloc(sizeof(int64_t) * (size_t)(n + 1)));
+  if (n > 0)
+    prefix_count_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, p, q, pl.as<int32_t>(),
+                                                             d_scratch + 1);
+  RS_TRY(exclusive_scan_i32_to_i64(pl.as<int32_t>(), n, poff.as<int64_t>(), tmp, st));
+  int64_t M = 0;
+  int32_t hm[2];
+  RS_HIP(hipMemcpyAsync(&M, poff.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(hm, d_scratch, sizeof(hm), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(!hm[1], "sss: a set has 2^21 or more tokens");
+  RS_TRY(ent.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
+  RS_TRY(ent_sorted.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
+  if (n > 0)
+    prefix_emit_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, ranked_sorted.as<int32_t>(),
+                                                            poff.as<int64_t>(), p, q, ent.as<uint64_t>());
+  if (M > 0) RS_TRY(sort_keys<uint64_t>(ent.as<uint64_t>(), ent_sorted.as<uint64_t>(), M, 64, tmp, st));
+  *M_out = M;
+  return RS_OK;
+}
+
+static rs_status unique_sorted(DevBuf& keys, int64_t C, DevBuf& tmp, hipStream_t st, DevBuf& uniq,
+                               int64_t* U_out);
+static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
+                                   const int32_t* b_el, DevBuf& uniq, int64_t U, int32_t p, int32_t q,
+                                   DevBuf& tmp, hipStream_t st, rs_sss_result* R);
+
 static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,
                           int32_t q, float alpha, int32_t bsz, hipStream_t st, rs_sss_result* R) {
   RS_REQUIRE(n >= 0 && n < (1ll << 21), "sss self-join: %lld sets exceed 2^21", (long long)n);
@@ -252,76 +351,20 @@ static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, 
   RS_TRY(scratch.alloc(16));
   int32_t* d_max = scratch.as<int32_t>();
   RS_HIP(hipMemsetAsync(scratch.p, 0, 16, st));
-  RS_HIP(hipMemsetAsync(d_max, 0xFF, sizeof(int32_t), st));
-  if (E > 0) max_token_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, d_max, d_max + 1);
-  int32_t hm[2];
-  RS_HIP(hipMemcpyAsync(hm, d_max, sizeof(hm), hipMemcpyDeviceToHost, st));
-  RS_HIP(hipStreamSynchronize(st));
-  RS_REQUIRE(!hm[1], "sss self-join: negative token id");
-  const int64_t V = (int64_t)hm[0] + 1;
-  RS_REQUIRE(V < (1ll << 22), "sss self-join: token ids must be < 2^22");
-  // 1-2. frequencies and ranks
-  DevBuf sorted_tok, freq, okeys, okeys_sorted, rank;
-  RS_TRY(sorted_tok.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
-  RS_TRY(freq.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
-  RS_HIP(hipMemsetAsync(freq.p, 0, sizeof(int32_t) * (size_t)(V > 0 ? V : 1), st));
-  if (E > 0) {
-    RS_TRY(sort_keys<int32_t>(d_el, sorted_tok.as<int32_t>(), E, bits_for((uint64_t)V), tmp, st));
-    run_freq_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(sorted_tok.as<int32_t>(), E, freq.as<int32_t>());
-  }
-  sorted_tok.release();
-  RS_TRY(okeys.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
-  RS_TRY(okeys_sorted.alloc(sizeof(uint64_t) * (size_t)(V > 0 ? V : 1)));
-  RS_TRY(rank.alloc(sizeof(int32_t) * (size_t)(V > 0 ? V : 1)));
-  if (V > 0) {
-    order_keys_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(freq.as<int32_t>(), V, okeys.as<uint64_t>());
-    RS_TRY(sort_keys<uint64_t>(okeys.as<uint64_t>(), okeys_sorted.as<uint64_t>(), V,
-                               32 + bits_for((uint64_t)(E > 0 ? E : 1)), tmp, st));
-    rank_kernel<<<grid_for(V, 256, 8), 256, 0, st>>>(okeys_sorted.as<uint64_t>(), V, rank.as<int32_t>());
-  }
-  okeys.release();
-  okeys_sorted.release();
-  freq.release();
-  // 3. each set in rank order
-  DevBuf ranked, ranked_sorted;
-  RS_TRY(ranked.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
-  RS_TRY(ranked_sorted.alloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1)));
-  if (E > 0) {
-    map_rank_kernel<<<grid_for(E, 256, 8), 256, 0, st>>>(d_el, E, rank.as<int32_t>(), ranked.as<int32_t>());
-    size_t bytes = 0;
-    RS_ROC(rocprim::segmented_radix_sort_keys(nullptr, bytes, ranked.as<int32_t>(),
-                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
-                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
-    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
-    RS_ROC(rocprim::segmented_radix_sort_keys(tmp.p, bytes, ranked.as<int32_t>(),
-                                              ranked_sorted.as<int32_t>(), (size_t)E, (unsigned)n,
-                                              d_off, d_off + 1, 0, bits_for((uint64_t)V), st));
-  }
-  ranked.release();
+  // 1-3. global order and each set in rank order
+  DevBuf rank, ranked_sorted;
+  int64_t V = 0;
+  RS_TRY(token_ranks(d_el, E, tmp, st, d_max, rank, &V));
+  RS_TRY(ranked_sets(d_off, n, d_el, E, rank, V, tmp, st, ranked_sorted));
   rank.release();
   RS_HIP(hipEventRecord(ev[1], st));
   // 4. prefix entries
-  DevBuf pl, poff, ent, ent_sorted;
-  RS_TRY(pl.alloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1)));
-  RS_TRY(poff.alloc(sizeof(int64_t) * (size_t)(n + 1)));
-  if (n > 0)
-    prefix_count_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, p, q, pl.as<int32_t>(), d_max + 1);
-  RS_TRY(exclusive_scan_i32_to_i64(pl.as<int32_t>(), n, poff.as<int64_t>(), tmp, st));
+  DevBuf ent_sorted;
   int64_t M = 0;
-  RS_HIP(hipMemcpyAsync(&M, poff.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  RS_HIP(hipMemcpyAsync(hm, d_max, sizeof(hm), hipMemcpyDeviceToHost, st));
-  RS_HIP(hipStreamSynchronize(st));
-  RS_REQUIRE(!hm[1], "sss self-join: a set has 2^21 or more tokens");
-  RS_TRY(ent.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
-  RS_TRY(ent_sorted.alloc(sizeof(uint64_t) * (size_t)(M > 0 ? M : 1)));
-  if (n > 0)
-    prefix_emit_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(d_off, n, ranked_sorted.as<int32_t>(),
-                                                            poff.as<int64_t>(), p, q, ent.as<uint64_t>());
-  if (M > 0) RS_TRY(sort_keys<uint64_t>(ent.as<uint64_t>(), ent_sorted.as<uint64_t>(), M, 64, tmp, st));
-  ent.release();
+  RS_TRY(prefix_entries(d_off, n, ranked_sorted, p, q, tmp, st, d_max, ent_sorted, &M));
   ranked_sorted.release();
   // 5. candidates
-  DevBuf cnt, coff, cand, cand_sorted, uniq, nuniq;
+  DevBuf cnt, coff, cand, uniq;
   RS_TRY(cnt.alloc(sizeof(int64_t) * (size_t)(M > 0 ? M : 1)));
   RS_TRY(coff.alloc(sizeof(int64_t) * (size_t)(M + 1)));
   if (M > 0)
@@ -337,79 +380,283 @@ static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, 
     return RS_ENOMEM;
   }
   cnt.release();
-  RS_TRY(cand.alloc(sizeof(uint64_t) * (size_t)(C > 0 ? C : 1)));
-  if (C > 0)
+  int64_t U = 0;
+  if (C > 0) {
+    RS_TRY(cand.alloc(sizeof(uint64_t) * (size_t)C));
     cand_emit_kernel<<<grid_for(M, 256, 8), 256, 0, st>>>(ent_sorted.as<uint64_t>(), M,
                                                           coff.as<int64_t>(), cand.as<uint64_t>());
-  ent_sorted.release();
-  coff.release();
-  int64_t U = 0;
-  RS_TRY(nuniq.alloc(16));
-  if (C > 0) {
-    RS_TRY(cand_sorted.alloc(sizeof(uint64_t) * (size_t)C));
-    RS_TRY(sort_keys<uint64_t>(cand.as<uint64_t>(), cand_sorted.as<uint64_t>(), C, 64, tmp, st));
-    cand.release();
-    RS_TRY(uniq.alloc(sizeof(uint64_t) * (size_t)C));
-    size_t bytes = 0;
-    RS_ROC(rocprim::unique(nullptr, bytes, cand_sorted.as<uint64_t>(), uniq.as<uint64_t>(),
-                           nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
-    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
-    RS_ROC(rocprim::unique(tmp.p, bytes, cand_sorted.as<uint64_t>(), uniq.as<uint64_t>(),
-                           nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
-    size_t hu = 0;
-    RS_HIP(hipMemcpyAsync(&hu, nuniq.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
-    RS_HIP(hipStreamSynchronize(st));
-    U = (int64_t)hu;
-    cand_sorted.release();
+    ent_sorted.release();
+    coff.release();
+    RS_TRY(unique_sorted(cand, C, tmp, st, uniq, &U));
   }
   R->num_candidates = U;
   RS_HIP(hipEventRecord(ev[2], st));
   // 6. verification: every candidate's |S_i cap S_j| by the batched LDS
   //    hash kernel (indexed pairs), then the exact rule + compaction
-  int64_t K = 0;
-  if (U > 0) {
-    DevBuf cu, cv, flags, sizes, out_n, work;
-    RS_TRY(cu.alloc(sizeof(int32_t) * (size_t)U));
-    RS_TRY(cv.alloc(sizeof(int32_t) * (size_t)U));
-    RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)U));
-    RS_TRY(sizes.alloc(sizeof(int32_t) * (size_t)U));
-    RS_TRY(work.alloc(16));
-    RS_HIP(hipMemsetAsync(work.p, 0, 16, st));
-    split_pairs_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(uniq.as<uint64_t>(), U, cu.as<int32_t>(),
-                                                           cv.as<int32_t>());
-    rs_pairs_plan* plan = nullptr;
-    RS_TRY(rs_pairs_plan_create(U, nullptr, &plan));
-    struct PGuard {
-      rs_pairs_plan* p;
-      ~PGuard() { rs_pairs_plan_free(p); }
-    } pg{plan};
-    RS_TRY(rs_pairs_intersect_indexed_async(plan, d_off, d_el, cu.as<int32_t>(), cv.as<int32_t>(), U,
-                                            sizes.as<int32_t>(), st));
-    jaccard_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(d_off, cu.as<int32_t>(), cv.as<int32_t>(),
-                                                             sizes.as<int32_t>(), U, p, q,
-                                                             flags.as<int32_t>(),
-                                                             work.as<unsigned long long>());
-    RS_TRY(R->pairs.alloc(sizeof(uint64_t) * (size_t)U));
-    RS_TRY(out_n.alloc(16));
-    size_t bytes = 0;
-    RS_ROC(rocprim::select(nullptr, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
-                           R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
-    if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
-    RS_ROC(rocprim::select(tmp.p, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
-                           R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
-    size_t hk = 0;
-    unsigned long long hw[2] = {0, 0};
-    RS_HIP(hipMemcpyAsync(&hk, out_n.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
-    RS_HIP(hipMemcpyAsync(hw, work.p, sizeof(hw), hipMemcpyDeviceToHost, st));
-    RS_HIP(hipStreamSynchronize(st));
-    K = (int64_t)hk;
-    R->verify_probes = (int64_t)hw[0];
-    R->verify_elements = (int64_t)hw[1];
-    RS_TRY(rs_pairs_plan_status(plan, st));
-  } else {
-    RS_TRY(R->pairs.alloc(16));
+  RS_TRY(verify_candidates(d_off, d_el, d_off, d_el, uniq, U, p, q, tmp, st, R));
+  RS_HIP(hipEventRecord(ev[3], st));
+  RS_HIP(hipEventSynchronize(ev[3]));
+  for (int i = 0; i < 3; ++i) RS_HIP(hipEventElapsedTime(&R->ms[i], ev[i], ev[i + 1]));
+  return RS_OK;
+}
+
+// ---------------------------------------------------------------- lookups
+// Query token keys: 1 + global rank for a token of the collection's id range,
+// 0 for any other token (frequency 0 there: it orders before every known
+// token and has no prefix entries).  A wave per query also checks the query
+// is strictly ascending, non-negative and shorter than 2^21 tokens.
+__global__ void query_keys_kernel(const int32_t* __restrict__ qoff, int64_t Q,
+                                  const int32_t* __restrict__ qtok, const int32_t* __restrict__ rank,
+                                  int64_t V, uint32_t* __restrict__ keys, int32_t* err) {
+  const int lane = __lane_id();
+  const int64_t waves = (int64_t)gridDim.x * blockDim.x / kWave;
+  bool bad = false;
+  for (int64_t qi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; qi < Q; qi += waves) {
+    const int32_t a = qoff[qi], z = qoff[qi + 1];
+    bad |= z < a || z - a >= (1 << 21);
+    for (int32_t i = a + lane; i < z; i += kWave) {
+      const int32_t t = qtok[i];
+      bad |= t < 0 || (i > a && qtok[i - 1] >= t);
+      keys[i] = (t >= 0 && t < V) ? (uint32_t)rank[t] + 1u : 0u;
+    }
   }
-  R->num_pairs = K;
+  if (__any(bad) && lane == 0) atomicOr(err, kErrBadInput);
+}
+
+// Per query: prefix length and the size window [ceil(p s / q), floor(q s / p)].
+__device__ __forceinline__ void lookup_window(int32_t s, int32_t p, int32_t q, uint64_t& smin,
+                                              uint64_t& smax) {
+  smin = ((uint64_t)p * (uint64_t)s + (uint64_t)q - 1) / (uint64_t)q;
+  smax = ((uint64_t)q * (uint64_t)s) / (uint64_t)p;
+  if (smin < 1) smin = 1;
+  if (smax > kField21) smax = kField21;
+}
+
+__global__ void lookup_prefix_len_kernel(const int32_t* __restrict__ qoff, int64_t Q, int32_t p,
+                                         int32_t q, int32_t* __restrict__ pl) {
+  for (int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < Q;
+       qi += (int64_t)gridDim.x * blockDim.x)
+    pl[qi] = prefix_len(qoff[qi + 1] - qoff[qi], p, q);
+}
+
+// For prefix position k of query qi: the run of prefix entries with the
+// token's rank and a size inside the window -> [lo, lo + cnt).
+__global__ void lookup_count_kernel(const int32_t* __restrict__ qoff, int64_t Q,
+                                    const uint32_t* __restrict__ skeys, const int64_t* __restrict__ poff,
+                                    const uint64_t* __restrict__ ent, int64_t M, int32_t p, int32_t q,
+                                    int64_t* __restrict__ lo_out, int64_t* __restrict__ cnt_out) {
+  const int lane = __lane_id();
+  const int64_t waves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t qi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; qi < Q; qi += waves) {
+    const int32_t a = qoff[qi], s = qoff[qi + 1] - a;
+    const int64_t p0 = poff[qi], np = poff[qi + 1] - p0;
+    uint64_t smin, smax;
+    lookup_window(s, p, q, smin, smax);
+    for (int64_t k = lane; k < np; k += kWave) {
+      const uint32_t key = skeys[a + k];
+      int64_t lo = 0, cnt = 0;
+      if (key != 0u && smin <= smax) {
+        const uint64_t r = key - 1u;
+        lo = lower_bound_u64(ent, 0, M, (r << kRankShift) | (smin << kSizeShift));
+        const int64_t hi = lower_bound_u64(ent, lo, M, ((r << kRankShift) | (smax << kSizeShift) | kField21) + 1);
+        cnt = hi - lo;
+      }
+      lo_out[p0 + k] = lo;
+      cnt_out[p0 + k] = cnt;
+    }
+  }
+}
+
+__global__ void lookup_emit_kernel(const int64_t* __restrict__ poff, int64_t Q,
+                                   const int64_t* __restrict__ lo, const int64_t* __restrict__ coff,
+                                   const uint64_t* __restrict__ ent, uint64_t* __restrict__ cand) {
+  const int lane = __lane_id();
+  const int64_t waves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t qi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; qi < Q; qi += waves) {
+    for (int64_t k = poff[qi] + lane; k < poff[qi + 1]; k += kWave) {
+      const int64_t c0 = coff[k], c = coff[k + 1] - c0;
+      for (int64_t j = 0; j < c; ++j)
+        cand[c0 + j] = ((uint64_t)qi << 32) | (ent[lo[k] + j] & kField21);
+    }
+  }
+}
+
+static rs_status unique_sorted(DevBuf& keys, int64_t C, DevBuf& tmp, hipStream_t st, DevBuf& uniq,
+                               int64_t* U_out) {
+  DevBuf sorted, nuniq;
+  RS_TRY(sorted.alloc(sizeof(uint64_t) * (size_t)C));
+  RS_TRY(sort_keys<uint64_t>(keys.as<uint64_t>(), sorted.as<uint64_t>(), C, 64, tmp, st));
+  keys.release();
+  RS_TRY(uniq.alloc(sizeof(uint64_t) * (size_t)C));
+  RS_TRY(nuniq.alloc(16));
+  size_t bytes = 0;
+  RS_ROC(rocprim::unique(nullptr, bytes, sorted.as<uint64_t>(), uniq.as<uint64_t>(),
+                         nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
+  if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+  RS_ROC(rocprim::unique(tmp.p, bytes, sorted.as<uint64_t>(), uniq.as<uint64_t>(),
+                         nuniq.as<size_t>(), (size_t)C, rocprim::equal_to<uint64_t>(), st));
+  size_t hu = 0;
+  RS_HIP(hipMemcpyAsync(&hu, nuniq.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  *U_out = (int64_t)hu;
+  return RS_OK;
+}
+
+// Verification of unique candidate keys (a << 32 | b) over families A, B:
+// sizes by the batched LDS kernel, the exact rule, compaction into R.
+static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
+                                   const int32_t* b_el, DevBuf& uniq, int64_t U, int32_t p, int32_t q,
+                                   DevBuf& tmp, hipStream_t st, rs_sss_result* R) {
+  if (U == 0) {
+    RS_TRY(R->pairs.alloc(16));
+    R->num_pairs = 0;
+    return RS_OK;
+  }
+  DevBuf cu, cv, flags, sizes, out_n, work;
+  RS_TRY(cu.alloc(sizeof(int32_t) * (size_t)U));
+  RS_TRY(cv.alloc(sizeof(int32_t) * (size_t)U));
+  RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)U));
+  RS_TRY(sizes.alloc(sizeof(int32_t) * (size_t)U));
+  RS_TRY(work.alloc(16));
+  RS_HIP(hipMemsetAsync(work.p, 0, 16, st));
+  split_pairs_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(uniq.as<uint64_t>(), U, cu.as<int32_t>(),
+                                                         cv.as<int32_t>());
+  rs_pairs_plan* plan = nullptr;
+  RS_TRY(rs_pairs_plan_create(U, nullptr, &plan));
+  struct PGuard {
+    rs_pairs_plan* p;
+    ~PGuard() { rs_pairs_plan_free(p); }
+  } pg{plan};
+  RS_TRY(rs_pairs_intersect_cross_async(plan, a_off, a_el, b_off, b_el, cu.as<int32_t>(),
+                                        cv.as<int32_t>(), U, sizes.as<int32_t>(), st));
+  jaccard_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(a_off, b_off, cu.as<int32_t>(), cv.as<int32_t>(),
+                                                           sizes.as<int32_t>(), U, p, q,
+                                                           flags.as<int32_t>(),
+                                                           work.as<unsigned long long>());
+  RS_TRY(R->pairs.alloc(sizeof(uint64_t) * (size_t)U));
+  RS_TRY(out_n.alloc(16));
+  size_t bytes = 0;
+  RS_ROC(rocprim::select(nullptr, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
+                         R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
+  if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+  RS_ROC(rocprim::select(tmp.p, bytes, uniq.as<uint64_t>(), flags.as<int32_t>(),
+                         R->pairs.as<uint64_t>(), out_n.as<size_t>(), (size_t)U, st));
+  size_t hk = 0;
+  unsigned long long hw[2] = {0, 0};
+  RS_HIP(hipMemcpyAsync(&hk, out_n.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(hw, work.p, sizeof(hw), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  R->num_pairs = (int64_t)hk;
+  R->verify_probes = (int64_t)hw[0];
+  R->verify_elements = (int64_t)hw[1];
+  return rs_pairs_plan_status(plan, st);
+}
+
+static rs_status index_build(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,
+                             int32_t q, hipStream_t st, rs_sss_index* X) {
+  RS_REQUIRE(n >= 0 && n < (1ll << 21), "rs_sss_index_build: %lld sets exceed 2^21", (long long)n);
+  RS_REQUIRE(p > 0 && q >= p, "rs_sss_index_build: threshold p/q must be in (0, 1]");
+  int32_t E32 = 0;
+  if (n > 0) {
+    RS_HIP(hipMemcpyAsync(&E32, d_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  X->off = d_off;
+  X->el = d_el;
+  X->n = n;
+  X->E = E32;
+  X->p = p;
+  X->q = q;
+  DevBuf tmp, scratch, ranked_sorted;
+  RS_TRY(scratch.alloc(16));
+  RS_HIP(hipMemsetAsync(scratch.p, 0, 16, st));
+  RS_TRY(token_ranks(d_el, X->E, tmp, st, scratch.as<int32_t>(), X->rank, &X->V));
+  RS_TRY(ranked_sets(d_off, n, d_el, X->E, X->rank, X->V, tmp, st, ranked_sorted));
+  RS_TRY(prefix_entries(d_off, n, ranked_sorted, p, q, tmp, st, scratch.as<int32_t>(), X->ent, &X->M));
+  RS_HIP(hipStreamSynchronize(st));
+  return RS_OK;
+}
+
+static rs_status lookup(const rs_sss_index* X, const int32_t* d_qoff, int64_t Q, const int32_t* d_qtok,
+                        hipStream_t st, rs_sss_result* R) {
+  RS_REQUIRE(Q >= 0 && Q < (1ll << 31), "rs_sss_lookup: bad query count");
+  hipEvent_t ev[4];
+  for (auto& e : ev) RS_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  RS_HIP(hipEventRecord(ev[0], st));
+  int32_t EQ = 0;
+  if (Q > 0) {
+    RS_HIP(hipMemcpyAsync(&EQ, d_qoff + Q, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  RS_REQUIRE(EQ >= 0, "rs_sss_lookup: negative query offsets");
+  RS_REQUIRE(EQ == 0 || d_qtok, "rs_sss_lookup: null query tokens");
+  DevBuf tmp, scratch, keys, skeys, pl, poff;
+  RS_TRY(scratch.alloc(16));
+  RS_HIP(hipMemsetAsync(scratch.p, 0, 16, st));
+  RS_TRY(keys.alloc(sizeof(uint32_t) * (size_t)(EQ > 0 ? EQ : 1)));
+  RS_TRY(skeys.alloc(sizeof(uint32_t) * (size_t)(EQ > 0 ? EQ : 1)));
+  RS_TRY(pl.alloc(sizeof(int32_t) * (size_t)(Q > 0 ? Q : 1)));
+  RS_TRY(poff.alloc(sizeof(int64_t) * (size_t)(Q + 1)));
+  if (Q > 0) {
+    query_keys_kernel<<<grid_for(Q * kWave, 256, 8), 256, 0, st>>>(d_qoff, Q, d_qtok, X->rank.as<int32_t>(),
+                                                                   X->V, keys.as<uint32_t>(),
+                                                                   scratch.as<int32_t>());
+    int32_t herr = 0;
+    RS_HIP(hipMemcpyAsync(&herr, scratch.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+    RS_REQUIRE(!herr, "rs_sss_lookup: a query is not strictly ascending, has a negative token or "
+                      "2^21 or more tokens");
+    if (EQ > 0) {
+      size_t bytes = 0;
+      const unsigned bits = bits_for((uint64_t)X->V + 1);
+      RS_ROC(rocprim::segmented_radix_sort_keys(nullptr, bytes, keys.as<uint32_t>(), skeys.as<uint32_t>(),
+                                                (size_t)EQ, (unsigned)Q, d_qoff, d_qoff + 1, 0, bits, st));
+      if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+      RS_ROC(rocprim::segmented_radix_sort_keys(tmp.p, bytes, keys.as<uint32_t>(), skeys.as<uint32_t>(),
+                                                (size_t)EQ, (unsigned)Q, d_qoff, d_qoff + 1, 0, bits, st));
+    }
+    lookup_prefix_len_kernel<<<grid_for(Q, 256, 8), 256, 0, st>>>(d_qoff, Q, X->p, X->q, pl.as<int32_t>());
+  }
+  keys.release();
+  RS_TRY(exclusive_scan_i32_to_i64(pl.as<int32_t>(), Q, poff.as<int64_t>(), tmp, st));
+  int64_t NP = 0;
+  RS_HIP(hipMemcpyAsync(&NP, poff.as<int64_t>() + Q, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_HIP(hipEventRecord(ev[1], st));
+  DevBuf lo, cnt, coff, cand, uniq;
+  RS_TRY(lo.alloc(sizeof(int64_t) * (size_t)(NP > 0 ? NP : 1)));
+  RS_TRY(cnt.alloc(sizeof(int64_t) * (size_t)(NP > 0 ? NP : 1)));
+  RS_TRY(coff.alloc(sizeof(int64_t) * (size_t)(NP + 1)));
+  if (NP > 0)
+    lookup_count_kernel<<<grid_for(Q * kWave, 256, 8), 256, 0, st>>>(
+        d_qoff, Q, skeys.as<uint32_t>(), poff.as<int64_t>(), X->ent.as<uint64_t>(), X->M, X->p, X->q,
+        lo.as<int64_t>(), cnt.as<int64_t>());
+  RS_TRY(exclusive_scan_i64(cnt.as<int64_t>(), NP, coff.as<int64_t>(), tmp, st));
+  int64_t C = 0;
+  RS_HIP(hipMemcpyAsync(&C, coff.as<int64_t>() + NP, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  if (C > (1ll << 32)) {
+    set_error("rs_sss_lookup: %lld candidate pairs (before dedup) exceed the 2^32 working limit",
+              (long long)C);
+    return RS_ENOMEM;
+  }
+  int64_t U = 0;
+  if (C > 0) {
+    RS_TRY(cand.alloc(sizeof(uint64_t) * (size_t)C));
+    lookup_emit_kernel<<<grid_for(Q * kWave, 256, 8), 256, 0, st>>>(poff.as<int64_t>(), Q, lo.as<int64_t>(),
+                                                                    coff.as<int64_t>(), X->ent.as<uint64_t>(),
+                                                                    cand.as<uint64_t>());
+    RS_TRY(unique_sorted(cand, C, tmp, st, uniq, &U));
+  }
+  R->num_candidates = U;
+  RS_HIP(hipEventRecord(ev[2], st));
+  RS_TRY(verify_candidates(d_qoff, d_qtok, X->off, X->el, uniq, U, X->p, X->q, tmp, st, R));
   RS_HIP(hipEventRecord(ev[3], st));
   RS_HIP(hipEventSynchronize(ev[3]));
   for (int i = 0; i < 3; ++i) RS_HIP(hipEventElapsedTime(&R->ms[i], ev[i], ev[i + 1]));
@@ -462,5 +709,36 @@ rs_status rs_sss_result_timings(const rs_sss_result* r, float* h_ms3) {
 }
 
 void rs_sss_result_free(rs_sss_result* r) { delete r; }
+
+rs_status rs_sss_index_build(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                             int32_t p, int32_t q, rs_stream stream, rs_sss_index** out) {
+  RS_REQUIRE(out && d_offsets && (d_elements || num_sets == 0), "rs_sss_index_build: null argument");
+  *out = nullptr;
+  auto* X = new rs_sss_index();
+  rs_status s = index_build(d_offsets, num_sets, d_elements, p, q, as_stream(stream), X);
+  if (s != RS_OK) {
+    delete X;
+    return s;
+  }
+  *out = X;
+  return RS_OK;
+}
+
+void rs_sss_index_free(rs_sss_index* idx) { delete idx; }
+
+rs_status rs_sss_lookup(const rs_sss_index* idx, const int32_t* d_query_offsets,
+                        int64_t num_queries, const int32_t* d_query_tokens, rs_stream stream,
+                        rs_sss_result** out) {
+  RS_REQUIRE(idx && out && d_query_offsets, "rs_sss_lookup: null argument");
+  *out = nullptr;
+  auto* R = new rs_sss_result();
+  rs_status s = lookup(idx, d_query_offsets, num_queries, d_query_tokens, as_stream(stream), R);
+  if (s != RS_OK) {
+    delete R;
+    return s;
+  }
+  *out = R;
+  return RS_OK;
+}
 
 }  // extern "C"

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreset_amd.so")
 
 RS_OK, RS_EINVAL, RS_ENOMEM, RS_EOVERFLOW, RS_EDEVICE, RS_EBUSY = 0, 1, 2, 3, 4, 5
+REQ_IR, REQ_SSS = 0, 1
 NORMAL, HIERARCHICAL = 0, 1
 BUILD_NORMAL, BUILD_HIERARCHICAL, BUILD_BOTH = 1, 2, 3
 PAIRS_MAX_BINS = 20
@@ -44,9 +45,10 @@ class rs_pairs_opts(C.Structure):
                 ("waves_per_cu", C.c_int32), ("lds_slots_per_wave", C.c_int32)]
 
 
-class rs_ir_completion(C.Structure):
+class rs_completion(C.Structure):
     _fields_ = [("query_id", C.c_int64), ("count", C.c_int32), ("status", C.c_int32),
-                ("submit_ms", C.c_double), ("done_ms", C.c_double)]
+                ("submit_ms", C.c_double), ("done_ms", C.c_double), ("kind", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 _p, _i32, _i64, _u32, _u64, _f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
@@ -78,20 +80,26 @@ SIGNATURES = {
     "rs_sss_result_timings": (_i32, [_p, _p]),
     "rs_sss_result_work": (_i32, [_p, _p, _p]),
     "rs_sss_result_free": (None, [_p]),
+    "rs_sss_index_build": (_i32, [_p, _i64, _p, _i32, _i32, _p, _pp]),
+    "rs_sss_index_free": (None, [_p]),
+    "rs_sss_lookup": (_i32, [_p, _p, _i64, _p, _p, _pp]),
     "rs_ir_query_async": (_i32, [_p, _i32, _p, _p, _p, _i64, _p, _p, _p, _p, _p]),
-    "rs_ir_server_create": (_i32, [_p, _i32, _i64, _i32, _i32, _pp]),
-    "rs_ir_server_submit": (_i32, [_p, _i64, _p, _i32]),
-    "rs_ir_server_poll": (_i32, [_p, _p, _i32, _p]),
-    "rs_ir_server_result": (_i32, [_p, _i64, _p, _i64, _p]),
-    "rs_ir_server_drain": (_i32, [_p]),
-    "rs_ir_server_stats": (_i32, [_p, _p, _p, _p]),
-    "rs_ir_server_free": (None, [_p]),
+    "rs_server_create": (_i32, [_p, _i32, _i64, _i32, _i32, _pp]),
+    "rs_server_submit_ir": (_i32, [_p, _i64, _p, _i32]),
+    "rs_server_attach_sss": (_i32, [_p, _p]),
+    "rs_server_submit_sss": (_i32, [_p, _i64, _p, _i32]),
+    "rs_server_poll": (_i32, [_p, _p, _i32, _p]),
+    "rs_server_result": (_i32, [_p, _i64, _p, _i64, _p]),
+    "rs_server_drain": (_i32, [_p]),
+    "rs_server_stats": (_i32, [_p, _p, _p, _p]),
+    "rs_server_free": (None, [_p]),
     "rs_pairs_default_opts": (None, [C.POINTER(rs_pairs_opts)]),
     "rs_pairs_intersect_plain_async": (_i32, [_p, _p, _p, _p, _p, _p, _i64, _f32, _i32, _p, _p, _p]),
     "rs_pairs_plan_create": (_i32, [_i64, C.POINTER(rs_pairs_opts), _pp]),
     "rs_pairs_plan_free": (None, [_p]),
     "rs_pairs_intersect_batched_async": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
     "rs_pairs_intersect_indexed_async": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
+    "rs_pairs_intersect_cross_async": (_i32, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p]),
     "rs_pairs_graph_capture": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),
     "rs_pairs_graph_launch": (_i32, [_p, _p]),
     "rs_pairs_plan_status": (_i32, [_p, _p]),

@@ -203,3 +203,63 @@ def test_ir_server_memory_admission_splits_batches(api, oracle_lib):
     assert [got[i] for i in range(120)] == want
     assert stats["launches"] >= 20
     t.close()
+
+
+def test_server_mixes_ir_queries_and_similarity_lookups(api, oracle_lib):
+    """IR queries and similarity lookups interleaved on one serving loop:
+    every result equals the request served alone and the oracle's."""
+    from reset_amd import serving
+
+    rng = np.random.default_rng(31)
+    idx_off, docs, q, qoff = _ir_case(rng, nterms=300, ndocs=200000, maxlen=20000, nq=120, kmax=6)
+    queries = [q[qoff[i]:qoff[i + 1]].tolist() for i in range(120)]
+    m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
+    want_ir = oracle_lib.ir_result_lists(m_res, m_cnt, m_roff)
+    soff, scols = G.zipf_sets(rng, 3000, 500, 3, 40)
+    lookups = []
+    for i in rng.choice(3000, 60, replace=False).tolist():
+        r = scols[soff[i]:soff[i + 1]].tolist()
+        if len(r) > 3 and i % 2:
+            r = sorted(set(r[1:] + [600 + i]))  # a near copy with an unknown token
+        lookups.append(r)
+    lookups.append([])
+    lq = np.zeros(len(lookups) + 1, np.int64)
+    np.cumsum([len(r) for r in lookups], out=lq[1:])
+    ltok = np.concatenate([np.array(r, np.int32) for r in lookups])
+    keys = oracle_lib.sss_lookup_bruteforce(soff, scols, lq.astype(np.int32), ltok, 4, 5)
+    want_sss = [[k & 0xFFFFFFFF for k in keys.tolist() if k >> 32 == a] for a in range(len(lookups))]
+    assert sum(len(w) for w in want_sss) >= 30
+    t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 1)
+    sidx = api.SSSIndex(_cuda(soff), _cuda(scols), 4, 5)
+    srv = serving.Server(t, budget_bytes=4 << 20, max_batch=16, max_inflight=3, sss_index=sidx)
+    n_ir, n_s = len(queries), len(lookups)
+    order = rng.permutation(n_ir + n_s).tolist()
+    for k in order:
+        if k < n_ir:
+            srv.submit_ir(k, queries[k])
+        else:
+            srv.submit_sss(k, lookups[k - n_ir])
+    srv.drain()
+    comps = srv.poll_all()
+    assert len(comps) == n_ir + n_s and all(c.status == 0 for c in comps)
+    kinds = {c.query_id: c.kind for c in comps}
+    got = {c.query_id: srv.result(c.query_id) for c in comps}
+    assert [got[i] for i in range(n_ir)] == want_ir
+    assert [got[n_ir + a] for a in range(n_s)] == want_sss
+    assert all(kinds[i] == serving.REQ_IR for i in range(n_ir))
+    assert all(kinds[n_ir + a] == serving.REQ_SSS for a in range(n_s))
+    st = srv.stats()
+    assert st["queries"] == n_ir + n_s and st["launches"] > 2
+    srv.close()
+    # each lookup alone, on a fresh server
+    srv = serving.Server(t, sss_index=sidx)
+    for a in [0, 1, 7, n_s - 1]:
+        srv.submit_sss(10_000 + a, lookups[a])
+        srv.drain()
+        assert srv.result(10_000 + a) == want_sss[a]
+    srv.close()
+    with pytest.raises(Exception):
+        bad = serving.Server(t)
+        bad.submit_sss(1, [1, 2, 3])  # no index attached
+    sidx.close()
+    t.close()

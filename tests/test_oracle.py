@@ -302,3 +302,18 @@ def test_ir_special_cases(oracle_lib):
     t = oracle_lib.build_tables(idx_off, docs, 0.2, 5, 0)
     res, cnt, roff = oracle_lib.ir_hash(idx_off, docs, t, q, qoff)
     assert oracle_lib.ir_result_lists(res, cnt, roff) == [[1, 5, 9], [], [5, 9]]
+
+
+def test_sss_lookup_bruteforce_agrees_with_selfjoin(oracle_lib):
+    """Looking every row up against the collection finds itself and exactly
+    the self-join partners, in both orders."""
+    rng = np.random.default_rng(21)
+    off, cols = G.zipf_sets(rng, 300, 200, 3, 25)
+    for p, q in [(4, 5), (1, 2)]:
+        pairs = oracle_lib.sss_bruteforce(off, cols, p, q)
+        want = {(i, i) for i in range(len(off) - 1) if off[i + 1] > off[i]}
+        for k in pairs.tolist():
+            i, j = k >> 32, k & 0xFFFFFFFF
+            want |= {(i, j), (j, i)}
+        got = oracle_lib.sss_lookup_bruteforce(off, cols, off, cols, p, q)
+        assert [(k >> 32, k & 0xFFFFFFFF) for k in got.tolist()] == sorted(want)
