@@ -227,8 +227,9 @@ def bench_tc(args):
         "pairs_per_s": edges / (ms / 1000.0),
         "variants_ms": {k: v[0] for k, v in res.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tc_staged_kernel + tc_tiled_kernel",
-                     "kernel_ms": ms, "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _tc_traffic(args.scale, alg_bytes),
+                     "kernel": "tc_staged_kernel + tc_tiled_kernel", "kernel_ms": ms,
+                     "alg_bytes_per_launch": alg_bytes},
         "prep_s": prep_s,
     }
     if not args.no_cpu:
@@ -247,6 +248,15 @@ def bench_tc(args):
                                    "bit_exact_vs_gpu": tri_cpu == tri}
     t.close()
     return out
+
+
+def _tc_traffic(scale, alg_bytes):
+    """Counter-measured HBM bytes per count when a profile of this workload
+    exists (same algorithmic bytes, i.e. the same graph)."""
+    tr = _traffic(f"tc_rmat{scale}")
+    if tr and tr.get("alg_bytes_per_launch") == alg_bytes:
+        return tr["hbm_bytes_per_launch"]
+    return None
 
 
 def bench_sss(args):
