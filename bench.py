@@ -59,6 +59,15 @@ def bench_pairs(args):
     alg_bytes = 4 * n_elems + 20 * P
 
     opts = api.default_pairs_opts()
+    tune = None
+    if not args.no_autotune:
+        # per-bin (load factor, chunk) by timing candidates on a sample (not timed)
+        S = min(P, args.tune_sample)
+        t1 = time.time()
+        opts = api.autotune_pairs(a_off[:S + 1].contiguous(), a, b_off[:S + 1].contiguous(), b, opts, reps=2)
+        _sync_all()
+        tune = {"sample_pairs": S, "tune_s": time.time() - t1,
+                "bins_alpha_b_chunk_unroll": api.pairs_opts_table(opts)}
     plan = api.PairsPlan(P, opts)
     sizes = torch.empty(P, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
@@ -119,6 +128,7 @@ def bench_pairs(args):
             "alg_bytes_per_launch": alg_bytes,
         },
         "gen_s": gen_s,
+        "autotune": tune,
     }
     tr = _traffic("pairs_batched")
     if tr and tr.get("pairs") == P:
@@ -259,6 +269,33 @@ def _tc_traffic(scale, alg_bytes):
     return None
 
 
+def _plant_near_duplicates(off, cols, count, vocab=1_000_000):
+    """Replace the last `count` sets by copies of random earlier sets with one
+    token swapped for another vocabulary token (J >= 0.8 for sets of >= 9
+    tokens), so the join has pairs to report."""
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(SEED + 10)
+    offh = off.cpu().numpy().astype(np.int64)
+    n = len(offh) - 1
+    count = min(count, n // 2)
+    keep = int(offh[n - count])
+    tail = []
+    for src in rng.choice(n - count, count, replace=False).tolist():
+        r = cols[int(offh[src]):int(offh[src + 1])].cpu().numpy().copy()
+        x = int(rng.integers(0, vocab))
+        while x in set(r.tolist()):
+            x = int(rng.integers(0, vocab))
+        r[int(rng.integers(0, len(r)))] = x
+        tail.append(np.sort(r))
+    lens = np.array([len(r) for r in tail], np.int64)
+    new_off = offh.copy()
+    new_off[n - count + 1:] = keep + np.cumsum(lens)
+    new_cols = torch.cat([cols[:keep], torch.from_numpy(np.concatenate(tail).astype(np.int32)).to(cols.device)])
+    return torch.from_numpy(new_off.astype(np.int32)).to(off.device), new_cols
+
+
 def bench_sss(args):
     import numpy as np
     import torch
@@ -267,6 +304,8 @@ def bench_sss(args):
 
     t0 = time.time()
     off, cols = inputs.zipf_set_collection(args.sets, 1_000_000, seed=SEED + 4)
+    if args.sss_plant > 0:
+        off, cols = _plant_near_duplicates(off, cols, args.sss_plant)
     _sync_all()
     gen_s = time.time() - t0
     for _ in range(args.warmup):
@@ -293,7 +332,9 @@ def bench_sss(args):
         "unit": "candidate pairs verified/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-        "data": "synthetic Zipf sets (seeded on device)",
+        "data": "synthetic Zipf sets (seeded on device)"
+                + (f", last {args.sss_plant} sets replaced by one-token-perturbed copies of earlier sets"
+                   if args.sss_plant else ""),
         "config": {"workload": f"set-similarity self-join J>=0.8, {args.sets} sets, sizes Zipf(1.2) in [10,2000], "
                                "tokens Zipf(1.0) over 1M, prefix filter + batched hash verification",
                    "sets": args.sets, "elements": int(offh[-1]), "parallelism": "single GPU"},
@@ -479,6 +520,8 @@ def main(argv=None):
     ap.add_argument("--qps", type=float, default=2000.0)
     ap.add_argument("--ir-budget-mb", type=int, default=4096)
     ap.add_argument("--ir-batch", type=int, default=256)
+    ap.add_argument("--sss-plant", type=int, default=1000,
+                    help="sss workload: near-duplicate sets planted at the end of the collection")
     ap.add_argument("--sss-lookups", type=int, default=0,
                     help="ir workload: also serve this many similarity lookups mixed with the queries")
     ap.add_argument("--sss-qps", type=float, default=500.0)
@@ -490,6 +533,8 @@ def main(argv=None):
     ap.add_argument("--plain-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="pairs: keep the default per-bin settings")
+    ap.add_argument("--tune-sample", type=int, default=200_000, help="pairs: pairs the autotuner times on")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy of this many MiB (counter calibration)")
     args = ap.parse_args(argv)

@@ -281,6 +281,22 @@ def default_pairs_opts():
     return o
 
 
+def autotune_pairs(a_off, a, b_off, b, opts=None, reps=2, stream=None):
+    """Per-size-bin (load factor, chunk) picked by timing candidates on these
+    pairs (a sample of the workload); returns the tuned rs_pairs_opts."""
+    _need_cuda(a_off, a, b_off, b)
+    o = opts if opts is not None else default_pairs_opts()
+    check(load().rs_pairs_autotune(ptr(a_off), ptr(a), ptr(b_off), ptr(b), a_off.numel() - 1, int(reps),
+                                   C.byref(o), stream_handle(stream)))
+    return o
+
+
+def pairs_opts_table(o):
+    """The bins of an rs_pairs_opts as plain tuples (alpha, b, chunk, unroll)."""
+    return [(round(o.bins[i].load_factor, 4), o.bins[i].bucket_size, o.bins[i].chunk, o.bins[i].unroll)
+            for i in range(o.num_bins)]
+
+
 class PairsPlan:
     """Batched persistent pair-intersection launch (size-binned plan)."""
 

@@ -29,9 +29,9 @@ struct Cand {
 };
 
 std::vector<Cand> candidates() {
-  const float alphas[] = {0.2f, 0.25f, 0.5f, 0.75f, 0.9f};
+  const float alphas[] = {0.125f, 0.25f, 0.5f, 0.75f, 0.9f};
   const int bs[] = {4};
-  const int chunks[] = {255, 511, 1024};
+  const int chunks[] = {127, 255, 511, 1024};
   std::vector<Cand> out;
   for (float a : alphas)
     for (int b : bs)
@@ -88,7 +88,11 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
     float best = 1e30f;
     rs_pairs_bin best_bin = opts->bins[bin];
     best_bin.unroll = opts->bins[0].unroll;
-    for (const Cand& c : cands) {
+    // the bin's current setting competes too, so tuning never picks a slower one
+    std::vector<Cand> tries;
+    tries.push_back(Cand{best_bin, 0.0});
+    tries.insert(tries.end(), cands.begin(), cands.end());
+    for (const Cand& c : tries) {
       rs_pairs_opts trial = *opts;
       for (int k = 0; k < nb; ++k) {
         trial.bins[k] = c.bin;
@@ -108,6 +112,7 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
       if (ms_min < best) {
         best = ms_min;
         best_bin = c.bin;
+        best_bin.unroll = opts->bins[0].unroll;
       }
     }
     opts->bins[bin] = best_bin;

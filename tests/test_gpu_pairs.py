@@ -175,3 +175,24 @@ def test_indexed_pairs_match_oracle(api, oracle_lib):
     assert np.array_equal(sizes.cpu().numpy(), want)
     assert np.array_equal(sizes.cpu().numpy()[:50], np.diff(off)[u[:50]])
     plan.close()
+
+
+def test_autotuned_opts_stay_bit_exact(api, oracle_lib):
+    """The autotuner's per-bin choice keeps results exact (the tuned plan is
+    checked against the constructed overlaps and the oracle)."""
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(3000, seed=77, lo=16, hi=20000)
+    o = api.autotune_pairs(batch["a_off"], batch["a"], batch["b_off"], batch["b"], reps=1)
+    table = api.pairs_opts_table(o)
+    assert all(b == 4 and 1 <= c <= 1024 and 0 < al <= 1 for al, b, c, _ in table)
+    plan = api.PairsPlan(3000, o)
+    sizes = torch.empty(3000, dtype=torch.int32, device="cuda")
+    plan.run_async(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes)
+    torch.cuda.synchronize()
+    plan.status()
+    assert torch.equal(sizes, batch["k"])
+    ha, hb = batch["a_off"].cpu().numpy(), batch["b_off"].cpu().numpy()
+    ref = oracle_lib.pairs_merge(ha, batch["a"].cpu().numpy(), hb, batch["b"].cpu().numpy())
+    assert np.array_equal(ref, sizes.cpu().numpy())
+    plan.close()
