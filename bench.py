@@ -349,6 +349,9 @@ def bench_sss(args):
         "gen_s": gen_s,
     }
     out["roofline"]["frac"] = out["roofline"]["achieved"] / HBM_PEAK_GBS
+    tr = _traffic("sss_selfjoin")
+    if tr and tr.get("candidates") == res["candidates"]:
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
     if not args.no_cpu:
         import oracle
 
