@@ -31,8 +31,8 @@ HIP_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
-SOURCES = ["core.hip", "tables.hip", "tc.hip", "pairs.hip", "sss.hip", "sss_join.hip", "ir.hip", "gen.hip",
-           "autotune.hip"]
+SOURCES = ["core.hip", "tables.hip", "tc.hip", "pairs.hip", "sss.hip", "sss_join.hip", "sss_group.hip", "ir.hip",
+           "gen.hip", "autotune.hip"]
 TOOLS = ["tc", "sss", "ir", "intersection"]
 
 

@@ -1,0 +1,190 @@
+// sss_group.hip -- grouped verification of similarity candidates.
+//
+// Candidates (u[i], v[i]) arrive sorted by u, and a set with a rare prefix
+// token is a candidate of hundreds of others, so the pairs of one u form long
+// runs.  A run (cut into units of at most kGroupCands candidates) is verified
+// by one workgroup: it inserts S_u into an LDS table once (bucket-major,
+// kGroupB-slot buckets, load factor kGroupAlpha), then its waves stream the
+// partner sets S_v through it, one candidate per wave at a time.  Per pair
+// the cost is the streamed set alone; the build is paid once per unit.
+//
+// Probe rule (the same walk as TC.cu:11-31, a bucket at a time): inserts take
+// the first EMPTY slot of their walk and never clear one, so a bucket's
+// occupied slots are a prefix of it; a key is in its home bucket, or, if that
+// bucket is full, further along -- the walk stops at a bucket with an EMPTY
+// slot.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "sss_verify.hpp"
+
+namespace rs {
+
+constexpr int kGroupBlock = 256;
+constexpr int kGroupWaves = kGroupBlock / kWave;
+constexpr int kGroupB = 4;
+constexpr float kGroupAlpha = 0.25f;
+constexpr int64_t kGroupCands = 1024;
+constexpr int kGroupU = 4;  // streamed keys per lane per step
+
+// Largest build set the grouped path takes: calculate_length(2047, 1/4, 4) =
+// 2048 buckets x 4 slots = 32 KiB of LDS per workgroup.
+constexpr int32_t kGroupMaxBuild = 2047;
+constexpr int32_t kGroupSlots = 2048 * kGroupB;
+
+__device__ __forceinline__ void group_bucket(const int32_t* __restrict__ tab, uint32_t j, int32_t x,
+                                             bool& hit, bool& full) {
+  const int4 v = *reinterpret_cast<const int4*>(tab + j * kGroupB);
+  uint32_t m = min(min((uint32_t)(v.x ^ x), (uint32_t)(v.y ^ x)),
+                   min((uint32_t)(v.z ^ x), (uint32_t)(v.w ^ x)));
+  hit = m == 0u;
+  full = v.w != kEmpty;
+}
+
+// unit_start[k] = first candidate of unit k: a candidate opens a unit when
+// its u differs from the previous one's or its index is a multiple of
+// kGroupCands.
+__global__ void group_flags_kernel(const int32_t* __restrict__ cu, int64_t n,
+                                   int32_t* __restrict__ flags) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = i == 0 || cu[i] != cu[i - 1] || i % kGroupCands == 0;
+}
+
+__global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
+    const int32_t* __restrict__ a_off, const int32_t* __restrict__ a_el,
+    const int32_t* __restrict__ b_off, const int32_t* __restrict__ b_el,
+    const int32_t* __restrict__ cu, const int32_t* __restrict__ cv,
+    const int64_t* __restrict__ unit_start, int64_t nunits, int64_t ncand,
+    int32_t* __restrict__ sizes, int32_t* __restrict__ err) {
+  __shared__ __attribute__((aligned(16))) int32_t tab[kGroupSlots];
+  const int lane = __lane_id();
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  bool bad = false;
+  for (int64_t k = blockIdx.x; k < nunits; k += gridDim.x) {
+    const int64_t c0 = unit_start[k];
+    const int64_t c1 = k + 1 < nunits ? unit_start[k + 1] : ncand;
+    const int32_t u = cu[c0];
+    const int32_t a0 = a_off[u], na = a_off[u + 1] - a0;
+    const uint32_t L = (uint32_t)calc_length(na, kGroupAlpha, kGroupB);
+    const uint32_t nslots = L * kGroupB;  // <= kGroupSlots (build sets are <= kGroupMaxBuild)
+    for (uint32_t s = threadIdx.x; s < nslots / 4; s += kGroupBlock)
+      *reinterpret_cast<int4*>(tab + 4 * s) = int4{kEmpty, kEmpty, kEmpty, kEmpty};
+    __syncthreads();
+    for (int32_t t = threadIdx.x; t < na; t += kGroupBlock) {
+      const int32_t x = a_el[a0 + t];
+      uint32_t slot = ((uint32_t)x & (L - 1u)) * kGroupB;
+      uint32_t step = 0;
+      for (; step < nslots; ++step) {
+        if (atomicCAS(&tab[slot], kEmpty, x) == kEmpty) break;
+        slot = slot + 1u == nslots ? 0u : slot + 1u;
+      }
+      bad |= step == nslots;
+    }
+    __syncthreads();
+    for (int64_t i = c0 + wid; i < c1; i += kGroupWaves) {
+      const int32_t v = cv[i];
+      const int32_t b0 = b_off[v], nb = b_off[v + 1] - b0;
+      const int32_t* __restrict__ G = b_el + b0;
+      uint32_t hits = 0;
+      for (int32_t it = 0; it < nb; it += kWave * kGroupU) {
+        int32_t x[kGroupU];
+        bool act[kGroupU], pend[kGroupU];
+        uint32_t j[kGroupU];
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < kGroupU; ++q) {
+          const int32_t e = it + q * kWave + lane;
+          act[q] = e < nb;
+          x[q] = G[act[q] ? e : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < kGroupU; ++q) {
+          j[q] = (uint32_t)x[q] & (L - 1u);
+          bool hit, full;
+          group_bucket(tab, j[q], x[q], hit, full);
+          hits += act[q] && hit;
+          pend[q] = act[q] && !hit && full;
+          any |= pend[q];
+        }
+        if (__any(any)) {  // rare: walk on past full home buckets
+#pragma unroll
+          for (int q = 0; q < kGroupU; ++q) {
+            bool p = pend[q];
+            uint32_t jj = j[q];
+            for (uint32_t step = 1; step < L && __any(p); ++step) {
+              jj = (jj + 1u) & (L - 1u);
+              bool hit, full;
+              group_bucket(tab, jj, x[q], hit, full);
+              hits += p && hit;
+              p = p && !hit && full;
+            }
+            bad |= p;
+          }
+        }
+      }
+      const uint32_t total = wave_sum(hits);
+      if (lane == 0) sizes[i] = (int32_t)total;
+    }
+    __syncthreads();
+  }
+  if (__any(bad) && lane == 0) atomicOr(err, kErrTableFull);
+}
+
+// Which candidates the grouped path takes: build set no larger than
+// kGroupMaxBuild.  sel[i] = 1 for those.
+__global__ void group_select_kernel(const int32_t* __restrict__ a_off, const int32_t* __restrict__ cu,
+                                    int64_t n, int32_t* __restrict__ sel) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t u = cu[i];
+    sel[i] = a_off[u + 1] - a_off[u] <= kGroupMaxBuild;
+  }
+}
+
+#define RS_ROC_G(expr)                                                \
+  do {                                                                \
+    hipError_t _e = (expr);                                           \
+    if (_e != hipSuccess) {                                           \
+      ::rs::set_error("%s failed: %s", #expr, hipGetErrorString(_e)); \
+      return RS_EDEVICE;                                              \
+    }                                                                 \
+  } while (0)
+
+rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
+                       const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t n,
+                       int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st) {
+  if (n == 0) return RS_OK;
+  DevBuf flags, starts, nsel;
+  RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)n));
+  RS_TRY(starts.alloc(sizeof(int64_t) * (size_t)n));
+  RS_TRY(nsel.alloc(16));
+  group_flags_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(cu, n, flags.as<int32_t>());
+  rocprim::counting_iterator<int64_t> idx(0);
+  size_t bytes = 0;
+  RS_ROC_G(rocprim::select(nullptr, bytes, idx, flags.as<int32_t>(), starts.as<int64_t>(),
+                           nsel.as<size_t>(), (size_t)n, st));
+  if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+  RS_ROC_G(rocprim::select(tmp.p, bytes, idx, flags.as<int32_t>(), starts.as<int64_t>(),
+                           nsel.as<size_t>(), (size_t)n, st));
+  size_t nunits = 0;
+  RS_HIP(hipMemcpyAsync(&nunits, nsel.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  const int grid = device_cu_count() * 4;  // 32 KiB of LDS each: 4 workgroups per CU
+  group_verify_kernel<<<grid, kGroupBlock, 0, st>>>(a_off, a_el, b_off, b_el, cu, cv,
+                                                    starts.as<int64_t>(), (int64_t)nunits, n, sizes,
+                                                    d_err);
+  RS_HIP(hipGetLastError());
+  RS_HIP(hipStreamSynchronize(st));  // flags/starts are released on return
+  return RS_OK;
+}
+
+rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t* sel,
+                          hipStream_t st) {
+  if (n > 0) group_select_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(a_off, cu, n, sel);
+  RS_HIP(hipGetLastError());
+  return RS_OK;
+}
+
+}  // namespace rs

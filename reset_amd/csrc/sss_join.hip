@@ -502,6 +502,106 @@ static rs_status unique_sorted(DevBuf& keys, int64_t C, DevBuf& tmp, hipStream_t
   return RS_OK;
 }
 
+__global__ void gather_i32_kernel(const int32_t* __restrict__ src, const int64_t* __restrict__ idx,
+                                  int64_t n, int32_t* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+__global__ void scatter_i32_kernel(const int32_t* __restrict__ src, const int64_t* __restrict__ idx,
+                                   int64_t n, int32_t* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[idx[i]] = src[i];
+}
+
+__global__ void invert_flags_kernel(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = !in[i];
+}
+
+// Indices i < n with flags[i] != 0, ascending.
+static rs_status select_indices(const int32_t* flags, int64_t n, DevBuf& out, int64_t* count,
+                                DevBuf& tmp, hipStream_t st) {
+  DevBuf nsel;
+  RS_TRY(out.alloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1)));
+  RS_TRY(nsel.alloc(16));
+  rocprim::counting_iterator<int64_t> idx(0);
+  size_t bytes = 0;
+  RS_ROC(rocprim::select(nullptr, bytes, idx, flags, out.as<int64_t>(), nsel.as<size_t>(), (size_t)n, st));
+  if (tmp.bytes < bytes) RS_TRY(tmp.alloc(bytes));
+  RS_ROC(rocprim::select(tmp.p, bytes, idx, flags, out.as<int64_t>(), nsel.as<size_t>(), (size_t)n, st));
+  size_t h = 0;
+  RS_HIP(hipMemcpyAsync(&h, nsel.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  *count = (int64_t)h;
+  return RS_OK;
+}
+
+// sizes[i] = |A_{cu[i]} cap B_{cv[i]}| for candidates sorted by cu: the ones
+// whose build set fits the grouped path share one LDS table per run of equal
+// cu (sss_group.hip); the rest go through the batched pair kernel.
+static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
+                                 const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t U,
+                                 int32_t* sizes, DevBuf& tmp, hipStream_t st) {
+  DevBuf sel, err, idx_g;
+  RS_TRY(sel.alloc(sizeof(int32_t) * (size_t)U));
+  RS_TRY(err.alloc(16));
+  RS_HIP(hipMemsetAsync(err.p, 0, 16, st));
+  RS_TRY(group_partition(a_off, cu, U, sel.as<int32_t>(), st));
+  int64_t G = 0;
+  RS_TRY(select_indices(sel.as<int32_t>(), U, idx_g, &G, tmp, st));
+  if (G == U) {
+    RS_TRY(group_verify(a_off, a_el, b_off, b_el, cu, cv, U, sizes, err.as<int32_t>(), tmp, st));
+  } else {
+    if (G > 0) {
+      DevBuf gu, gv, gs;
+      RS_TRY(gu.alloc(sizeof(int32_t) * (size_t)G));
+      RS_TRY(gv.alloc(sizeof(int32_t) * (size_t)G));
+      RS_TRY(gs.alloc(sizeof(int32_t) * (size_t)G));
+      const int g = grid_for(G, 256, 8);
+      gather_i32_kernel<<<g, 256, 0, st>>>(cu, idx_g.as<int64_t>(), G, gu.as<int32_t>());
+      gather_i32_kernel<<<g, 256, 0, st>>>(cv, idx_g.as<int64_t>(), G, gv.as<int32_t>());
+      RS_TRY(group_verify(a_off, a_el, b_off, b_el, gu.as<int32_t>(), gv.as<int32_t>(), G,
+                          gs.as<int32_t>(), err.as<int32_t>(), tmp, st));
+      scatter_i32_kernel<<<g, 256, 0, st>>>(gs.as<int32_t>(), idx_g.as<int64_t>(), G, sizes);
+    }
+    const int64_t R = U - G;
+    DevBuf inv, idx_r, ru, rv, rs;
+    RS_TRY(inv.alloc(sizeof(int32_t) * (size_t)U));
+    invert_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(sel.as<int32_t>(), U, inv.as<int32_t>());
+    int64_t R2 = 0;
+    RS_TRY(select_indices(inv.as<int32_t>(), U, idx_r, &R2, tmp, st));
+    RS_REQUIRE(R2 == R, "sss: candidate split is inconsistent");
+    RS_TRY(ru.alloc(sizeof(int32_t) * (size_t)R));
+    RS_TRY(rv.alloc(sizeof(int32_t) * (size_t)R));
+    RS_TRY(rs.alloc(sizeof(int32_t) * (size_t)R));
+    const int g = grid_for(R, 256, 8);
+    gather_i32_kernel<<<g, 256, 0, st>>>(cu, idx_r.as<int64_t>(), R, ru.as<int32_t>());
+    gather_i32_kernel<<<g, 256, 0, st>>>(cv, idx_r.as<int64_t>(), R, rv.as<int32_t>());
+    rs_pairs_plan* plan = nullptr;
+    RS_TRY(rs_pairs_plan_create(R, nullptr, &plan));
+    struct PGuard {
+      rs_pairs_plan* p;
+      ~PGuard() { rs_pairs_plan_free(p); }
+    } pg{plan};
+    RS_TRY(rs_pairs_intersect_cross_async(plan, a_off, a_el, b_off, b_el, ru.as<int32_t>(),
+                                          rv.as<int32_t>(), R, rs.as<int32_t>(), st));
+    scatter_i32_kernel<<<g, 256, 0, st>>>(rs.as<int32_t>(), idx_r.as<int64_t>(), R, sizes);
+    RS_TRY(rs_pairs_plan_status(plan, st));
+  }
+  int32_t herr = 0;
+  RS_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  if (herr) {
+    set_error("sss: a grouped verification table overflowed");
+    return RS_EOVERFLOW;
+  }
+  return RS_OK;
+}
+
 // Verification of unique candidate keys (a << 32 | b) over families A, B:
 // sizes by the batched LDS kernel, the exact rule, compaction into R.
 static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
@@ -521,14 +621,8 @@ static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, co
   RS_HIP(hipMemsetAsync(work.p, 0, 16, st));
   split_pairs_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(uniq.as<uint64_t>(), U, cu.as<int32_t>(),
                                                          cv.as<int32_t>());
-  rs_pairs_plan* plan = nullptr;
-  RS_TRY(rs_pairs_plan_create(U, nullptr, &plan));
-  struct PGuard {
-    rs_pairs_plan* p;
-    ~PGuard() { rs_pairs_plan_free(p); }
-  } pg{plan};
-  RS_TRY(rs_pairs_intersect_cross_async(plan, a_off, a_el, b_off, b_el, cu.as<int32_t>(),
-                                        cv.as<int32_t>(), U, sizes.as<int32_t>(), st));
+  RS_TRY(sizes_by_groups(a_off, a_el, b_off, b_el, cu.as<int32_t>(), cv.as<int32_t>(), U,
+                         sizes.as<int32_t>(), tmp, st));
   jaccard_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(a_off, b_off, cu.as<int32_t>(), cv.as<int32_t>(),
                                                            sizes.as<int32_t>(), U, p, q,
                                                            flags.as<int32_t>(),
@@ -549,7 +643,7 @@ static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, co
   R->num_pairs = (int64_t)hk;
   R->verify_probes = (int64_t)hw[0];
   R->verify_elements = (int64_t)hw[1];
-  return rs_pairs_plan_status(plan, st);
+  return RS_OK;
 }
 
 static rs_status index_build(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,

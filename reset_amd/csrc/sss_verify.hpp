@@ -13,4 +13,13 @@ rs_status sss_verify_enqueue(const rs_tables* t, int32_t layout, const int32_t* 
                              int32_t q, int32_t* flags, int32_t* sizes, DevBuf& W, DevBuf& tmp,
                              hipStream_t st);
 
+// Grouped verification (sss_group.hip): sizes[i] = |A_{cu[i]} cap B_{cv[i]}|
+// for candidates sorted by cu whose build set A_{cu[i]} has at most 2047
+// elements (group_partition flags those: sel[i] = 1).  Synchronizes st.
+rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
+                       const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t n,
+                       int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st);
+rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t* sel,
+                          hipStream_t st);
+
 }  // namespace rs

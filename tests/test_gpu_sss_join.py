@@ -74,3 +74,28 @@ def test_selfjoin_edge_cases(api):
     assert got["pairs"].cpu().tolist() == [(0 << 32) | 1]
     one = api.sss_selfjoin(_cuda(np.array([0, 2], np.int32)), _cuda(np.array([7, 9], np.int32)), 4, 5)
     assert one["pairs"].numel() == 0 and one["candidates"] == 0
+
+
+def test_selfjoin_mixes_grouped_and_batched_verification(api, oracle_lib):
+    """Sets above 2047 tokens cannot share the grouped LDS table: their
+    candidates take the batched pair kernel while the rest are grouped; the
+    joined result is the brute force's either way."""
+    rng = np.random.default_rng(41)
+    off, cols = G.zipf_sets(rng, 400, 300, 3, 40)
+    off, cols = _with_near_duplicates(rng, off, cols, 40)
+    rows = [cols[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+    nsmall = len(rows)
+    for i in range(12):
+        big = sorted(set(rng.integers(0, 200000, int(rng.integers(2100, 2600))).tolist()))
+        rows.append(big)
+        near = list(big)
+        near[int(rng.integers(0, len(near)))] = 300000 + i
+        rows.append(sorted(set(near)))
+    o = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum([len(r) for r in rows], out=o[1:])
+    off2, cols2 = o.astype(np.int32), np.concatenate([np.array(r, np.int32) for r in rows])
+    want = oracle_lib.sss_bruteforce(off2, cols2, 4, 5)
+    got = api.sss_selfjoin(_cuda(off2), _cuda(cols2), 4, 5)
+    assert np.array_equal(got["pairs"].cpu().numpy(), want)
+    big_pairs = [k for k in want.tolist() if (k >> 32) >= nsmall]
+    assert len(big_pairs) >= 12 and len(want) > len(big_pairs)
