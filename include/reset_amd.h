@@ -126,7 +126,7 @@ void rs_tc_plan_free(rs_tc_plan* p);
 /* Total probes (sum over edges of min(d+(u), d+(v))) and edges of the plan. */
 rs_status rs_tc_plan_work(const rs_tc_plan* p, int64_t* h_probes, int64_t* h_edges);
 /* The plan's LDS-staged share: probes whose table is copied into LDS once per
- * unit (tables probed >= 4x per slot), the number of units and the largest
+ * unit (tables probed >= 2x per slot), the number of units and the largest
  * staged table length.  Diagnostic; no counterpart in the reference (its
  * kernel reads every table slot from global memory, TC.cu:11-31). */
 rs_status rs_tc_plan_staged(const rs_tc_plan* p, int64_t* h_staged_probes, int64_t* h_units,

@@ -9,7 +9,7 @@
 // w in schedule order, so the probe space [0, W[E]) is flat.
 //
 // Staged form (tables probed at least kStageRatio times per slot, which at
-// R-MAT scale 20-24 carries ~97-99% of all probes): the schedule is sorted by
+// R-MAT scale 20-24 carries ~99% of all probes): the schedule is sorted by
 // (class, g, s), so each such table's probes are one contiguous range of the
 // probe space.  That range is cut into units; a workgroup copies the unit's
 // table (all b slot planes) from HBM into LDS once, then its waves walk the
@@ -61,11 +61,11 @@ namespace rs {
 constexpr int kTcBlock = 256;
 constexpr int64_t kTcTile = 4096;
 // A table is staged when its probes reach kStageRatio x its L*b slots.
-constexpr int64_t kStageRatio = 4;
+constexpr int64_t kStageRatio = 2;
 // Units hold at least max(kUnitProbes, kUnitSlotFactor * L*b) probes (a table
-// with fewer probes is one unit), so each copy serves >= 8 probes per slot.
-constexpr int64_t kUnitProbes = 32768;
-constexpr int64_t kUnitSlotFactor = 8;
+// with fewer probes is one unit), so each copy serves >= 16 probes per slot.
+constexpr int64_t kUnitProbes = 65536;
+constexpr int64_t kUnitSlotFactor = 16;
 // Probe tile of the staged form: one wave's share of a unit at a time.
 constexpr int64_t kStageTile = 2048;
 // Staged-table classes by length; waves per workgroup of each class.
