@@ -344,7 +344,7 @@ def bench_sss(args):
         "parity": {"exact_jaccard_of_reported_pairs": bool(ok)},
         "roofline": {"bound": "hbm", "achieved": alg / (vms / 1000.0) / 1e9 if vms > 0 else 0.0,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                     "kernel": "verify stage (batched pairs kernel over candidates + exact rule + compaction)",
+                     "kernel": "verify stage (group_verify_kernel over candidates + exact rule + compaction)",
                      "alg_bytes_per_launch": alg},
         "gen_s": gen_s,
     }
