@@ -64,6 +64,8 @@ def _declare(L):
     L.orc_tc_hash.argtypes = [_p, _p, _i64, _p, _p, _i32, _i32, _i32, _p, C.c_int, _p]
     L.orc_tc_merge.restype = C.c_uint64
     L.orc_tc_merge.argtypes = [_p, _p, _i64, C.c_int]
+    L.orc_tc_merge_rows.restype = C.c_uint64
+    L.orc_tc_merge_rows.argtypes = [_p, _p, _i64, _i64, C.c_int]
     L.orc_pairs_hash.restype = C.c_int
     L.orc_pairs_hash.argtypes = [_p, _p, _p, _p, _i64, _f32, _i32, C.c_int, _p]
     L.orc_pairs_merge.restype = None
@@ -204,6 +206,12 @@ def tc_merge(off, cols, nthreads=0) -> int:
     off = _c(off, np.int32)
     cols = _c(cols, np.int32)
     return int(lib().orc_tc_merge(_ptr(off), _ptr(cols), len(off) - 1, nthreads))
+
+
+def tc_merge_rows(off, cols, row_begin, row_end, nthreads=0) -> int:
+    """Triangles whose source vertex lies in [row_begin, row_end)."""
+    off, cols = _c(off, np.int32), _c(cols, np.int32)
+    return int(lib().orc_tc_merge_rows(_ptr(off), _ptr(cols), int(row_begin), int(row_end), nthreads))
 
 
 def pairs_hash(a_off, a, b_off, b, load_factor=0.2, bucket_size=5, nthreads=0):

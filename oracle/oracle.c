@@ -318,6 +318,20 @@ uint64_t orc_tc_merge(const int32_t* off, const int32_t* cols, int64_t n, int nt
   return total;
 }
 
+uint64_t orc_tc_merge_rows(const int32_t* off, const int32_t* cols, int64_t row_begin,
+                           int64_t row_end, int nthreads) {
+  uint64_t total = 0;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+  for (int64_t u = row_begin; u < row_end; u++)
+    for (int32_t e = off[u]; e < off[u + 1]; e++) {
+      const int32_t v = cols[e];
+      total += (uint64_t)merge_count(cols + off[u], off[u + 1] - off[u],
+                                     cols + off[v], off[v + 1] - off[v]);
+    }
+  return total;
+}
+
 /* set_size/intersection.cu:128-181, one pair at a time: build the larger set's
  * table (normal hash; with one table per launch BN = L), probe the smaller. */
 int orc_pairs_hash(const int64_t* a_off, const int32_t* a, const int64_t* b_off,

@@ -85,6 +85,13 @@ int orc_tc_hash(const int32_t* off, const int32_t* cols, int64_t n,
  * merge (rows must be sorted ascending). */
 uint64_t orc_tc_merge(const int32_t* off, const int32_t* cols, int64_t n,
                       int nthreads);
+/* The same count restricted to source rows [row_begin, row_end): in an
+ * acyclic orientation every triangle has one source (the vertex with both of
+ * its other vertices as out-neighbours), counted at its first out-edge, so
+ * the rows' count is the number of triangles whose source lies in the range
+ * and the counts of disjoint ranges add up to orc_tc_merge. */
+uint64_t orc_tc_merge_rows(const int32_t* off, const int32_t* cols, int64_t row_begin,
+                           int64_t row_end, int nthreads);
 
 /* set_size/intersection.cu:128-181 for many pairs: per pair, a table of the
  * larger set (length rule of hashtable.cu:20-36, single-table BN = L) probed

@@ -317,3 +317,17 @@ def test_sss_lookup_bruteforce_agrees_with_selfjoin(oracle_lib):
             want |= {(i, j), (j, i)}
         got = oracle_lib.sss_lookup_bruteforce(off, cols, off, cols, p, q)
         assert [(k >> 32, k & 0xFFFFFFFF) for k in got.tolist()] == sorted(want)
+
+
+def test_tc_row_ranges_add_up(oracle_lib):
+    """Disjoint source-row ranges partition the triangles (each is counted at
+    its source vertex)."""
+    rng = np.random.default_rng(12)
+    n = 300
+    und = G.undirected(G.random_graph(n, 5000, rng), n)
+    for by in ("degree", "id"):
+        off, cols = G.orient(und, n, by)
+        total = oracle_lib.tc_merge(off, cols)
+        cuts = [0, 37, 120, 121, 250, n]
+        parts = [oracle_lib.tc_merge_rows(off, cols, a, b) for a, b in zip(cuts, cuts[1:])]
+        assert sum(parts) == total == G.brute_triangles(und, n)
