@@ -349,6 +349,19 @@ rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_a, const in
                             const int32_t* d_b, int64_t num_pairs, int32_t reps,
                             rs_pairs_opts* opts, rs_stream stream);
 
+/* ------------------------------------------------------ hyperparameter model */
+/* The reference's analytic cost model (hyperparameter_solver/src/solver.py:
+ * 132-387): expected probe steps x step cost of a bucketed table of arr_size
+ * entries at load factor alpha with b-slot buckets, for warp width W,
+ * transaction bytes S_trans and slot bytes S_slot (binomial bucket loads,
+ * Markov overflow chain, occupancy by convolution).  Double precision. */
+double rs_hyper_cost(int64_t arr_size, double alpha, int32_t b, int32_t W, int32_t S_trans,
+                     int32_t S_slot);
+/* solve_hyperparameters (c_api.py:288-338): the least-cost grid point,
+ * alpha in {0.10, 0.15, ..., 1.00} and b in {1..9}. */
+rs_status rs_hyper_solve(int64_t arr_size, int32_t W, int32_t S_trans, int32_t S_slot,
+                         double* h_alpha, int32_t* h_b, double* h_cost);
+
 /* --------------------------------------------------------------- generators */
 /* Synthetic inputs (not on the measured path): counter-based, seeded,
  * reproducible.  Pair sets: A_i = sorted(pi_i[0 .. |A_i|)),

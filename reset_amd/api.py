@@ -281,6 +281,19 @@ def default_pairs_opts():
     return o
 
 
+def hyper_cost(arr_size, alpha, b, W=32, S_trans=32, S_slot=4):
+    """Analytic cost of (alpha, b) for a table of arr_size entries."""
+    return float(load().rs_hyper_cost(int(arr_size), float(alpha), int(b), int(W), int(S_trans), int(S_slot)))
+
+
+def hyper_solve(arr_size, W=32, S_trans=128, S_slot=4):
+    """Least-cost (alpha, b, cost) over the reference solver's grid."""
+    al, bb, co = C.c_double(0), C.c_int32(0), C.c_double(0)
+    check(load().rs_hyper_solve(int(arr_size), int(W), int(S_trans), int(S_slot), C.byref(al), C.byref(bb),
+                                C.byref(co)))
+    return float(al.value), int(bb.value), float(co.value)
+
+
 def autotune_pairs(a_off, a, b_off, b, opts=None, reps=2, stream=None):
     """Per-size-bin (load factor, chunk) picked by timing candidates on these
     pairs (a sample of the workload); returns the tuned rs_pairs_opts."""

@@ -106,6 +106,8 @@ SIGNATURES = {
     "rs_pairs_plan_timing": (_i32, [_p, _i32]),
     "rs_pairs_plan_kernel_ms": (_i32, [_p, C.POINTER(C.c_float)]),
     "rs_pairs_autotune": (_i32, [_p, _p, _p, _p, _i64, _i32, C.POINTER(rs_pairs_opts), _p]),
+    "rs_hyper_cost": (C.c_double, [_i64, C.c_double, _i32, _i32, _i32, _i32]),
+    "rs_hyper_solve": (_i32, [_i64, _i32, _i32, _i32, _p, _p, _p]),
     "rs_gen_pair_sets_async": (_i32, [_p, _p, _p, _i64, _u64, _p, _p, _p]),
     "rs_gen_rmat_async": (_i32, [_i32, _i64, _u32, _u32, _u32, _u64, _p, _p]),
 }
