@@ -83,3 +83,38 @@ def test_cost_sparse_closed_form():
     # swapping S_trans and S_slot changes T_step: ceil(1024 / 0.36) + 1 = 2846
     c = api.hyper_cost(1_000_000, 0.01, 9, 32, 4, 32)
     assert c == pytest.approx(1.09 * 2846, rel=1e-9)
+
+
+# res/sc18.csv, sc19.csv and sc20.csv hold the full grid of costs of the
+# reference's solver for the R-MAT scale-18/19/20 graphs at W = 32,
+# S_trans = 128, S_slot = 4, with the table size taken as the graph's
+# degree-weighted mean degree (hyperparameter_solver/src/util.py:131-146),
+# which the files do not record.  One row, (0.9, 4), fixes the size (its cost
+# rises monotonically with the size, one step at a time); the other 170 rows
+# of each grid must then match to the 6 printed decimals.
+GRID_SIZES = {"sc18": 1254, "sc19": 1683, "sc20": 2123}
+
+
+@pytest.mark.parametrize("sc", sorted(GRID_SIZES))
+def test_full_grid_matches_reference_csv(sc):
+    from concurrent.futures import ThreadPoolExecutor
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "solver_grid.json")) as f:
+        rows = json.load(f)["grids"][sc]
+    assert len(rows) == 19 * 9
+    assert sorted((a, b) for a, b, _ in rows) == sorted(
+        (round(0.1 + 0.05 * i, 2), b) for i in range(19) for b in range(1, 10))
+    n = GRID_SIZES[sc]
+    with ThreadPoolExecutor(8) as ex:
+        costs = list(ex.map(lambda r: api.hyper_cost(n, r[0], r[1], 32, 128, 4), rows))
+    for (a, b, ref), c in zip(rows, costs):
+        if ref >= 1e8:  # pruned points: the penalty constants themselves
+            assert c == ref, (a, b)
+        else:
+            assert abs(c - ref) <= 5.01e-7, (a, b, c, ref)
+    # the neighbouring sizes do not fit: the grid pins the size, and so the
+    # binomial (not Poisson) occupancy model
+    for m in (n - 1, n + 1):
+        assert abs(api.hyper_cost(m, 0.9, 4, 32, 128, 4) - dict(((a, b), c) for a, b, c in rows)[(0.9, 4)]) > 2e-6
+    # the recommendation of the reference's summary (res/*_summary.json)
+    assert api.hyper_solve(n, 32, 128, 4)[:2] == (0.2, 5)
