@@ -270,13 +270,16 @@ void rs_server_free(rs_server* s);
 /* ----------------------------------------------------------------------- a12 */
 /* Pairwise set intersection sizes for P pairs (A_i, B_i) given as two CSR
  * families with int64 offsets.  Sets need not be sorted for the hash probe;
- * the batched form's range chunking requires ascending sets (flag below). */
+ * the batched form requires ascending sets when the smaller set exceeds the
+ * bin's chunk (below). */
 typedef struct rs_pairs_bin {
   float load_factor;   /* alpha of the per-pair LDS table                  */
   int32_t bucket_size; /* b: slots per bucket, 1, 2 or 4 (one LDS read)    */
-  int32_t chunk;       /* max build-side elements per table, <= 1024       */
-  int32_t unroll;      /* 16-byte loads per lane in flight (1, 2, 4); the  */
-                       /* same in every bin                                */
+  int32_t chunk;       /* build-side elements of one table, 64..1024; a   */
+                       /* larger build side goes in chunks of <= 252      */
+  int32_t unroll;      /* 16-byte loads per lane in flight (1, 2, 4) where */
+                       /* |larger| >= 4 |smaller| (else 1); same in every  */
+                       /* bin                                              */
 } rs_pairs_bin;
 #define RS_PAIRS_MAX_BINS 20
 typedef struct rs_pairs_opts {
@@ -301,10 +304,11 @@ rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* 
 
 /* The batched form: one persistent, size-binned, work-stealing launch.  A
  * plan bins the pairs (largest first) once per batch; the launch builds each
- * pair's smaller set into an LDS table (range-chunked when it exceeds the
- * bin's chunk, which needs ascending sets) and streams the larger set through
- * it.  The plan + launch sequence contains no memset and can be replayed from
- * a hipGraph (rs_pairs_graph_*). */
+ * pair's smaller set into an LDS table and streams the larger set through
+ * it; a smaller set above the bin's chunk is built chunk by chunk in value
+ * order (ascending sets needed).
+ * The plan + launch sequence contains no memset and can be replayed from a
+ * hipGraph (rs_pairs_graph_*). */
 typedef struct rs_pairs_plan rs_pairs_plan;
 rs_status rs_pairs_plan_create(int64_t max_pairs, const rs_pairs_opts* opts, rs_pairs_plan** out);
 void rs_pairs_plan_free(rs_pairs_plan* p);

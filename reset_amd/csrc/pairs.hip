@@ -4,17 +4,15 @@
 // workspace table with -1, build the larger set into it, probe the smaller.
 //
 // Batched form: one persistent launch over all pairs.  Each wave owns a
-// private LDS table region and takes pairs from an atomic counter in the
-// order of a size-binned plan (largest bins first, so the longest pairs start
-// early and the small ones fill the tail).  Per pair it builds the SMALLER
-// set's table in LDS -- slot-plane layout of the reference with BN = L --
-// and streams the LARGER set through it once, so HBM sees each input element
-// exactly once (the algorithmic minimum 4(|A|+|B|) bytes).  A build side
-// larger than the bin's chunk is split into value ranges: chunk c covers
-// S[c0, c0+C) and the larger set is streamed in order, stopping at the first
-// element >= S[c0+C], where the next chunk's table takes over (both sets
-// ascending).  Everything stays integer; the per-pair count is reduced
-// across the wave and written once.
+// private LDS table region and takes the pairs of a size-binned plan (largest
+// bins first) round-robin.  Per pair it builds the SMALLER set into an LDS
+// table (bucket-major, the reference's bucket rule) and streams the LARGER
+// set through it once, so HBM sees each input element once (the algorithmic
+// minimum 4(|A|+|B|) bytes).  A smaller set above the bin's chunk is built
+// chunk by chunk in value order (both sets ascending), each chunk answering
+// for the larger set's elements up to its own last element; its elements are
+// staged through LDS ahead of use.  Everything stays integer; the per-pair
+// count is reduced across the wave and written once.
 #include "common.hpp"
 
 namespace rs {
@@ -187,95 +185,17 @@ __device__ __forceinline__ void lds_read_bucket(const int32_t* tab, uint32_t buc
   }
 }
 
-// N keys of one lane looked up together: every step reads one bucket per
-// pending key (N LDS reads in flight), resolving a key on a match or on an
-// EMPTY slot in its bucket; a full bucket without the key moves it on.
-template <int B, int N>
-__device__ __forceinline__ uint32_t lds_lookup_n(const int32_t* tab, uint32_t L,
-                                                 const int32_t (&x)[N], const bool (&act)[N],
-                                                 bool& bad) {
-  const uint32_t mask = L - 1u;
-  uint32_t bucket[N];
-  bool pend[N];
-  uint32_t hits = 0;
-#pragma unroll
-  for (int n = 0; n < N; ++n) {
-    bucket[n] = (uint32_t)x[n] & mask;
-    pend[n] = act[n];
-  }
-  for (uint32_t step = 0; step < L; ++step) {
-    int32_t v[N][B];
-#pragma unroll
-    for (int n = 0; n < N; ++n) lds_read_bucket<B>(tab, bucket[n], v[n]);
-    bool left = false;
-#pragma unroll
-    for (int n = 0; n < N; ++n) {
-      bool found = false, empty = false;
-#pragma unroll
-      for (int k = 0; k < B; ++k) {
-        found |= v[n][k] == x[n];
-        empty |= v[n][k] == kEmpty;
-      }
-      if (pend[n]) {
-        hits += found;
-        pend[n] = !(found || empty);
-        left |= pend[n];
-        bucket[n] = (bucket[n] + 1u) & mask;
-      }
-    }
-    if (!__any(left)) return hits;
-  }
-  bad = true;
-  return hits;
-}
-
-// Lookups when no key of the table was displaced: a key present in the table
-// is in its home bucket, so one bucket read decides it (no EMPTY test, no
-// walk).  any(slot == x) is evaluated as min(slot ^ x) == 0 in vector ALU.
-template <int B, int N>
-__device__ __forceinline__ uint32_t lds_lookup_home(const int32_t* tab, uint32_t L,
-                                                    const int32_t (&x)[N], const bool (&act)[N]) {
-  const uint32_t mask = L - 1u;
-  int32_t v[N][B];
-#pragma unroll
-  for (int n = 0; n < N; ++n) lds_read_bucket<B>(tab, (uint32_t)x[n] & mask, v[n]);
-  uint32_t hits = 0;
-#pragma unroll
-  for (int n = 0; n < N; ++n) {
-    uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
-#pragma unroll
-    for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
-    hits += (uint32_t)(m == 0u) & (uint32_t)act[n];
-  }
-  return hits;
-}
-
-// First index j in [lo, hi) with (uint32)G[j] >= key (G ascending), found
-// with 64-ary sampling: every round the 64 lanes test evenly spaced positions
-// and the ballot (a prefix of lanes) narrows the range ~65x.
-__device__ __forceinline__ int32_t wave_lower_bound(const int32_t* __restrict__ G, int32_t lo,
-                                                    int32_t hi, uint32_t key) {
-  const int lane = __lane_id();
-  while (hi - lo > kWave) {
-    const int64_t span = hi - lo;
-    const int32_t p = lo + (int32_t)((span * (lane + 1)) / (kWave + 1));
-    const uint64_t m = __ballot((uint32_t)G[p] < key);
-    const int c = __popcll(m);
-    const int32_t plo = __shfl(p, c > 0 ? c - 1 : 0, kWave);
-    const int32_t phi = __shfl(p, c < kWave ? c : kWave - 1, kWave);
-    if (c > 0) lo = plo + 1;
-    if (c < kWave) hi = phi;
-  }
-  const bool less = lo + lane < hi && (uint32_t)G[lo + lane] < key;
-  return lo + __popcll(__ballot(less));
-}
-
 constexpr int kMaxChunkPerLane = 16;  // chunk <= 1024 build-side elements
+constexpr int32_t kWideRatio = 4;     // |G| >= 4 |S|: stream U blocks per lane
 
 // One block of the larger set: 4*U elements per lane, 256*U per wave, from a
 // 16-byte aligned position `pos` (which may lie up to 3 elements before the
 // set's first element: those lanes are inactive).  Element e is valid if
-// first <= e < n.
+// first <= e < n.  Every lane issues one aligned 16-byte load per unroll
+// step, with its address clamped to the aligned group holding element n-1:
+// an aligned 16-byte group that holds an element of the set lies in one
+// page, so the load is in bounds even where part of it is not.  No branch
+// separates the loads, so the wait before the probe counts them exactly.
 template <int U>
 struct GBlock {
   int32_t x[4 * U];
@@ -286,18 +206,12 @@ template <int U>
 __device__ __forceinline__ void load_block(const int32_t* __restrict__ G, int32_t pos, int32_t first,
                                            int32_t n, GBlock<U>& blk) {
   const int lane = __lane_id();
+  const int32_t last4 = pos + ((n - 1 - pos) & ~3);  // aligned group of element n-1
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const int32_t e = pos + 4 * (u * kWave + lane);  // first element of this v4
-    v4i v = {kEmpty, kEmpty, kEmpty, kEmpty};
-    if (e + 3 < n && e >= first) {
-      v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(G + e));
-    } else {
-      if (e >= first && e < n) v.x = G[e];
-      if (e + 1 >= first && e + 1 < n) v.y = G[e + 1];
-      if (e + 2 >= first && e + 2 < n) v.z = G[e + 2];
-      if (e + 3 >= first && e + 3 < n) v.w = G[e + 3];
-    }
+    const int32_t e = pos + 4 * (u * kWave + lane);  // first element of this group
+    const int32_t el = e < last4 ? e : last4;
+    const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(G + el));
     blk.x[4 * u] = v.x;
     blk.x[4 * u + 1] = v.y;
     blk.x[4 * u + 2] = v.z;
@@ -349,97 +263,220 @@ __device__ __forceinline__ void load_chunk(const int32_t* __restrict__ S, int32_
   }
 }
 
-template <int B, int U>
-__device__ __forceinline__ uint32_t probe_block(const int32_t* tab, uint32_t L, bool displaced,
-                                                const int32_t (&x)[4 * U], const bool (&in)[4 * U],
-                                                bool& bad) {
-  if (!displaced) return lds_lookup_home<B, 4 * U>(tab, L, x, in);
-  // rare: some key of this chunk left its home bucket; walk key by key
+// Lookups of N keys per lane.  Buckets fill as prefixes (an insert takes the
+// first EMPTY slot of its walk and no slot is emptied while the table lives),
+// so one bucket read decides a key unless that bucket is full and some key of
+// the table was placed outside its home bucket (`displaced`); only such keys
+// walk on, bucket by bucket, until they meet themselves or a bucket with an
+// EMPTY slot.  any(slot == x) is evaluated as min(slot ^ x) == 0.
+template <int B, int N>
+__device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, bool displaced,
+                                               const int32_t (&x)[N], const bool (&act)[N],
+                                               bool& bad) {
+  const uint32_t mask = L - 1u;
+  int32_t v[N][B];
+  uint32_t j[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    j[n] = (uint32_t)x[n] & mask;
+    lds_read_bucket<B>(tab, j[n], v[n]);
+  }
   uint32_t hits = 0;
-  for (int i = 0; i < 4 * U; ++i) {
-    int32_t x1[1] = {x[i]};
-    bool a1[1] = {in[i]};
-    hits += lds_lookup_n<B, 1>(tab, L, x1, a1, bad);
+  bool pend[N];
+  bool any = false;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
+#pragma unroll
+    for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
+    const bool hit = m == 0u;
+    hits += (uint32_t)(hit && act[n]);
+    pend[n] = act[n] && !hit && v[n][B - 1] != kEmpty;
+    any |= pend[n];
+  }
+  if (displaced && __any(any)) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      bool p = pend[n];
+      uint32_t jj = j[n];
+      for (uint32_t step = 1; step < L && __any(p); ++step) {
+        jj = (jj + 1u) & mask;
+        int32_t w[B];
+        lds_read_bucket<B>(tab, jj, w);
+        uint32_t m = (uint32_t)(w[0] ^ x[n]);
+#pragma unroll
+        for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(w[k] ^ x[n]));
+        hits += (uint32_t)(p && m == 0u);
+        p = p && m != 0u && w[B - 1] != kEmpty;
+      }
+      bad |= p;
+    }
   }
   return hits;
 }
 
-// One pair inside one wave.  The larger set G is streamed exactly once as a
-// sequence of aligned blocks (the next block is loaded while the current one
-// is probed).  The smaller set S is built into the LDS table chunk by chunk;
-// chunk c covers the values [S[c0], S[c0+cl]), so each block element is
-// probed against the chunk whose range holds it.  When a block crosses into
-// the next chunk, the table is rebuilt from the next chunk's elements, which
-// were prefetched into registers, and the same block's remaining elements are
-// probed again -- chunk switches cost LDS work only, never a global round trip.
+// Aligned start of G: at most 3 elements before G[0], in the same allocation.
+__device__ __forceinline__ int32_t block_origin(const int32_t* G) {
+  return -(int32_t)((reinterpret_cast<uintptr_t>(G) >> 2) & 3u);
+}
+
+// A pair whose smaller set S fits the bin's table (ns <= chunk): S is built
+// once and the larger set G streams through it in aligned blocks (the next
+// block is loaded while the current one is probed).  S and G need not be
+// sorted.
 template <int B, int U, int CPL>
-__device__ __forceinline__ uint32_t pair_intersect_wave(const int32_t* __restrict__ S, int32_t ns,
-                                                        const int32_t* __restrict__ G, int32_t ng,
-                                                        int32_t* tab, float alpha, int32_t chunk,
-                                                        bool& bad) {
+__device__ __forceinline__ uint32_t pair_resident_wave(const int32_t* __restrict__ S, int32_t ns,
+                                                       const int32_t* __restrict__ G, int32_t ng,
+                                                       int32_t* tab, float alpha, bool& bad) {
   constexpr int32_t kBlock = 4 * U * kWave;
-  uint32_t hits = 0;
-  // current chunk [c0, c0+cl) with upper bound hi; the next chunk [n0, n0+nl)
-  // is prefetched into sn together with its own upper bound nhi
-  int32_t c0 = 0;
-  int32_t cl = ns < chunk ? ns : chunk;
-  int32_t sv[CPL], sn[CPL];
-  load_chunk<CPL>(S, 0, cl, sv);
-  bool last = cl >= ns;
-  uint32_t hi = last ? 0xFFFFFFFFu : (uint32_t)S[cl];
-  int32_t n0 = cl, nl = 0;
-  uint32_t nhi = 0xFFFFFFFFu;
-  if (!last) {
-    nl = ns - n0 < chunk ? ns - n0 : chunk;
-    load_chunk<CPL>(S, n0, nl, sn);
-    if (n0 + nl < ns) nhi = (uint32_t)S[n0 + nl];
-  }
-  // aligned start of G (at most 3 elements before G[0], same allocation)
-  const int32_t mis = (int32_t)((reinterpret_cast<uintptr_t>(G) >> 2) & 3u);
-  int32_t pos = -mis;
+  int32_t sv[CPL], spos[CPL];
+  load_chunk<CPL>(S, 0, ns, sv);
+  int32_t pos = block_origin(G);
   GBlock<U> cur;
   load_block<U>(G, pos, 0, ng, cur);
-  uint32_t L = (uint32_t)calc_length(cl, alpha, B);
-  int32_t spos[CPL];
-  bool displaced = build_chunk<B, CPL>(tab, L, sv, cl, spos, bad);
+  const uint32_t L = (uint32_t)calc_length(ns, alpha, B);
+  const bool displaced = build_chunk<B, CPL>(tab, L, sv, ns, spos, bad);
+  uint32_t hits = 0;
   while (true) {
+    // the next block's load is issued even past the end (clamped, all lanes
+    // inactive), so the wait before the probe never covers it
     GBlock<U> nxt;
     const bool more = pos + kBlock < ng;
-    if (more) load_block<U>(G, pos + kBlock, 0, ng, nxt);
-    while (true) {
-      bool in[4 * U];
-      bool rest = false;
-#pragma unroll
-      for (int i = 0; i < 4 * U; ++i) {
-        const bool below = (uint32_t)cur.x[i] < hi;
-        in[i] = cur.act[i] && below;
-        cur.act[i] = cur.act[i] && !below;
-        rest |= cur.act[i];
-      }
-      hits += probe_block<B, U>(tab, L, displaced, cur.x, in, bad);
-      if (last || !__any(rest)) break;
-      // the block reaches into the next chunk: switch tables, probe again
-      c0 = n0;
-      cl = nl;
-      hi = nhi;
-      last = c0 + cl >= ns;
-#pragma unroll
-      for (int r = 0; r < CPL; ++r) sv[r] = sn[r];
-      if (!last) {
-        n0 = c0 + cl;
-        nl = ns - n0 < chunk ? ns - n0 : chunk;
-        load_chunk<CPL>(S, n0, nl, sn);
-        nhi = n0 + nl < ns ? (uint32_t)S[n0 + nl] : 0xFFFFFFFFu;
-      }
-      L = (uint32_t)calc_length(cl, alpha, B);
-      clear_chunk<CPL>(tab, spos);
-      displaced = build_chunk<B, CPL>(tab, L, sv, cl, spos, bad);
-    }
+    load_block<U>(G, pos + kBlock, 0, ng, nxt);
+    hits += lds_lookup<B, 4 * U>(tab, L, displaced, cur.x, cur.act, bad);
     if (!more) break;
     pos += kBlock;
     cur = nxt;
   }
   clear_chunk<CPL>(tab, spos);
+  return hits;
+}
+
+// Staging of the smaller set: groups of 252 elements, each read by one
+// 16-byte load in lanes 0..62 (252 * 4 bytes is a multiple of 16, so every
+// group keeps the set's 16-byte aligned origin), kept in a two-slot ring in
+// LDS behind each wave's table.
+constexpr int32_t kGroupElems = 4 * (kWave - 1);
+constexpr int32_t kRingSlot = 4 * kWave;
+constexpr int32_t kRing = 2 * kRingSlot;  // int32 per wave
+
+// A pair whose smaller set S exceeds the bin's chunk (both sets ascending).
+// S is cut into chunks of c = min(chunk, 252) elements; chunk k's table
+// answers for the values up to its last element, so a block element x is
+// probed against the first chunk whose last element is >= x (the last chunk
+// takes everything above).  When a block reaches past the current chunk the
+// table is emptied (its keys un-inserted), rebuilt from the next chunk and
+// the rest of the block probed again.  G streams once in aligned blocks.
+//
+// Loads: the next block of G and the next unstaged group of S are loaded at
+// the top of every block, unconditionally (indices clamped to the set, lanes
+// past it masked), and the group is written to the ring at the bottom of the
+// block -- when the block registers turn over anyway -- if its slot is free.
+// Chunks are built from the ring, so neither a probe nor a rebuild waits on
+// a load issued after the block it works on; only a rebuild that outruns the
+// staging loads its group itself.
+template <int B, int U>
+__device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict__ S, int32_t ns,
+                                                      const int32_t* __restrict__ G, int32_t ng,
+                                                      int32_t* tab, int32_t* ring, float alpha,
+                                                      int32_t chunk, bool& bad) {
+  constexpr int32_t kBlock = 4 * U * kWave;
+  const int lane = __lane_id();
+  const int32_t c = chunk < kGroupElems ? chunk : kGroupElems;
+  const uint32_t L = (uint32_t)calc_length(c, alpha, B);
+  // S group g covers element indices [s0 + 252g, s0 + 252g + 252)
+  const int32_t s0 = block_origin(S);
+  const int32_t s_last4 = s0 + ((ns - 1 - s0) & ~3);
+  const int32_t ld = 4 * (lane < kWave - 1 ? lane : kWave - 2);
+  auto group_load = [&](int32_t g) {
+    const int32_t e = s0 + g * kGroupElems + ld;
+    return __builtin_nontemporal_load(reinterpret_cast<const v4i*>(S + (e < s_last4 ? e : s_last4)));
+  };
+  auto group_store = [&](int32_t g, const v4i& v) {
+    if (lane < kWave - 1) *reinterpret_cast<v4i*>(ring + (g & 1) * kRingSlot + 4 * lane) = v;
+  };
+  auto group_of = [&](int32_t idx) { return (idx - s0) / kGroupElems; };
+  auto ring_at = [&](int32_t idx) {
+    const int32_t g = (idx - s0) / kGroupElems;
+    return ring[(g & 1) * kRingSlot + (idx - s0 - g * kGroupElems)];
+  };
+  {
+    const v4i g0 = group_load(0), g1 = group_load(1);
+    group_store(0, g0);
+    group_store(1, g1);
+  }
+  int32_t sc = 2;  // next group to stage
+  // the current chunk [cb, ce); its table keys' slots in spos
+  int32_t cb = 0, ce = c < ns ? c : ns;
+  bool last = ce >= ns;
+  int32_t spos[4];
+  auto build = [&]() {
+    // stage whatever the chunk needs that is not staged yet
+    const int32_t gh = group_of(ce - 1);
+    while (sc <= gh) {
+      __builtin_amdgcn_wave_barrier();
+      group_store(sc, group_load(sc));
+      ++sc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    bool moved = false;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t idx = cb + r * kWave + lane;
+      spos[r] = -1;
+      if (idx < ce) {
+        spos[r] = lds_insert<B>(tab, L, ring_at(idx), moved);
+        bad |= spos[r] < 0;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return __any(moved);
+  };
+  bool displaced = build();
+  // largest value the current table answers for
+  uint32_t top = last ? 0xFFFFFFFFu : (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
+  int32_t pos = block_origin(G);
+  GBlock<U> cur;
+  load_block<U>(G, pos, 0, ng, cur);
+  uint32_t hits = 0;
+  while (true) {
+    GBlock<U> nxt;
+    const bool more = pos + kBlock < ng;
+    load_block<U>(G, pos + kBlock, 0, ng, nxt);  // unconditional, as above
+    const int32_t staged_g = sc;
+    const v4i staged = group_load(staged_g);     // unconditional, as above
+    while (true) {
+      bool in[4 * U];
+      bool rest = false;
+#pragma unroll
+      for (int i = 0; i < 4 * U; ++i) {
+        const bool below = (uint32_t)cur.x[i] <= top;
+        in[i] = cur.act[i] && below;
+        cur.act[i] = cur.act[i] && !below;
+        rest |= cur.act[i];
+      }
+      hits += lds_lookup<B, 4 * U>(tab, L, displaced, cur.x, in, bad);
+      if (last || !__any(rest)) break;
+      // the block reaches past this chunk: rebuild from the next one
+      clear_chunk<4>(tab, spos);
+      cb = ce;
+      ce = ns - cb < c ? ns : cb + c;
+      last = ce >= ns;
+      displaced = build();
+      top = last ? 0xFFFFFFFFu : (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
+    }
+    if (!more) break;
+    // stage the group loaded above if it is still the next one and its ring
+    // slot is free (the group two before it lies below the current chunk)
+    if (staged_g == sc && sc <= group_of(cb) + 1) {
+      __builtin_amdgcn_wave_barrier();
+      group_store(sc, staged);
+      ++sc;
+    }
+    pos += kBlock;
+    cur = nxt;
+  }
+  clear_chunk<4>(tab, spos);
   __builtin_amdgcn_wave_barrier();
   return hits;
 }
@@ -456,7 +493,9 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
   const int lane = __lane_id();
   // wave-uniform: pair index, offsets and loop bounds stay in SGPRs
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  int32_t* tab = lds_dyn + wid * slots_per_wave;
+  // per wave: the table region, then the staging ring of the window path
+  int32_t* tab = lds_dyn + wid * (slots_per_wave + kRing);
+  int32_t* ring = tab + slots_per_wave;
   // the region starts all EMPTY; every pair leaves it that way
   for (int32_t s4 = lane; s4 < (slots_per_wave >> 2); s4 += kWave)
     *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
@@ -497,7 +536,16 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     uint32_t hits = 0;
     if (ns > 0) {
       const PairBinDev bin = cfg.bins[pair_bin((int64_t)ns + ng, cfg.num_bins)];
-      hits = pair_intersect_wave<B, U, CPL>(S, ns, Gp, ng, tab, bin.alpha, bin.chunk, bad);
+      // U blocks per lane pay off where the larger set dominates (more bytes
+      // in flight per wave); near-equal sizes keep single blocks, whose chunk
+      // switches re-probe less
+      const bool wide = U > 1 && ng >= kWideRatio * ns;
+      if (ns <= bin.chunk)
+        hits = wide ? pair_resident_wave<B, U, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad)
+                    : pair_resident_wave<B, 1, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad);
+      else
+        hits = wide ? pair_chunked_wave<B, U>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad)
+                    : pair_chunked_wave<B, 1>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad);
     }
     const uint32_t total = wave_sum(hits);
     if (lane == 0) sizes[i] = (int32_t)total;
@@ -582,16 +630,16 @@ static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   RS_REQUIRE(o->lds_slots_per_wave >= 256 && o->lds_slots_per_wave <= 8192 &&
                  (o->lds_slots_per_wave & 3) == 0,
              "pairs: lds_slots_per_wave must be a multiple of 4 in [256, 8192]");
-  RS_REQUIRE((int64_t)o->lds_slots_per_wave * 4 * o->waves_per_cu <= 160 * 1024,
-             "pairs: %d waves x %d slots exceed the 160 KiB LDS of a CU", o->waves_per_cu,
-             o->lds_slots_per_wave);
+  RS_REQUIRE((int64_t)(o->lds_slots_per_wave + kRing) * 4 * o->waves_per_cu <= 160 * 1024,
+             "pairs: %d waves x (%d table + %d staging) slots exceed the 160 KiB LDS of a CU",
+             o->waves_per_cu, o->lds_slots_per_wave, kRing);
   cfg->num_bins = o->num_bins;
   for (int i = 0; i < o->num_bins; ++i) {
     const rs_pairs_bin& bn = o->bins[i];
     RS_REQUIRE(bn.load_factor > 0.f && (bn.bucket_size == 1 || bn.bucket_size == 2 ||
-                                        bn.bucket_size == 4) && bn.chunk >= 1 &&
+                                        bn.bucket_size == 4) && bn.chunk >= kWave &&
                    bn.chunk <= kMaxChunkPerLane * kWave,
-               "pairs: bin %d needs bucket_size in {1,2,4} and chunk in [1, %d]", i,
+               "pairs: bin %d needs bucket_size in {1,2,4} and chunk in [%d, %d]", i, kWave,
                kMaxChunkPerLane * kWave);
     RS_REQUIRE(bn.unroll == o->bins[0].unroll && (bn.unroll == 1 || bn.unroll == 2 || bn.unroll == 4),
                "pairs: unroll must be 1, 2 or 4 and the same in every bin (bin %d)", i);
@@ -628,7 +676,7 @@ static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const 
   for (int i = 0; i < o.num_bins; ++i) max_chunk = o.bins[i].chunk > max_chunk ? o.bins[i].chunk : max_chunk;
   const int cpl = (max_chunk + kWave - 1) / kWave;
   const int32_t spw = o.lds_slots_per_wave;
-  const size_t lds = sizeof(int32_t) * (size_t)spw * kWavesPerBlock;
+  const size_t lds = sizeof(int32_t) * (size_t)(spw + kRing) * kWavesPerBlock;
   const int B = o.bins[0].bucket_size, U = o.bins[0].unroll;
 #define RS_LAUNCH_BU(b_, u_) launch_bu<b_, u_, Src>(cpl, blocks, lds, src, P, order, ctl, sizes, cfg, spw, st)
   if (B == 4) {
@@ -697,8 +745,9 @@ void rs_pairs_default_opts(rs_pairs_opts* o) {
   for (int i = 0; i < RS_PAIRS_MAX_BINS; ++i) {
     // alpha 1/8, b 4: L = nextpow2 > 2*chunk, so a 255-element chunk fills
     // the 2048-slot region at <= 1/2 key per 4-slot bucket, where a home
-    // bucket overflows (forcing the walking lookup) in < 10% of chunks.
-    o->bins[i] = rs_pairs_bin{0.125f, 4, 255, 1};
+    // bucket overflows (forcing the walking lookup) in < 10% of chunks; two
+    // blocks per lane in flight where one set is >= 4x the other.
+    o->bins[i] = rs_pairs_bin{0.125f, 4, 255, 2};
   }
 }
 
