@@ -331,22 +331,31 @@ __device__ __forceinline__ uint32_t pair_resident_wave(const int32_t* __restrict
   constexpr int32_t kBlock = 4 * U * kWave;
   int32_t sv[CPL], spos[CPL];
   load_chunk<CPL>(S, 0, ns, sv);
-  int32_t pos = block_origin(G);
-  GBlock<U> cur;
-  load_block<U>(G, pos, 0, ng, cur);
   const uint32_t L = (uint32_t)calc_length(ns, alpha, B);
+  // three block buffers in fixed roles (the loop is unrolled by three): each
+  // step loads the block two ahead into the buffer probed in the step before
+  // and probes the block loaded two steps ago, so two blocks are in flight
+  // while one is probed and no loaded register is ever copied; loads past
+  // the end are clamped (all lanes inactive)
+  int32_t pos = block_origin(G);
+  GBlock<U> b0, b1, b2;
+  load_block<U>(G, pos, 0, ng, b0);
+  load_block<U>(G, pos + kBlock, 0, ng, b1);
   const bool displaced = build_chunk<B, CPL>(tab, L, sv, ns, spos, bad);
   uint32_t hits = 0;
   while (true) {
-    // the next block's load is issued even past the end (clamped, all lanes
-    // inactive), so the wait before the probe never covers it
-    GBlock<U> nxt;
-    const bool more = pos + kBlock < ng;
-    load_block<U>(G, pos + kBlock, 0, ng, nxt);
-    hits += lds_lookup<B, 4 * U>(tab, L, displaced, cur.x, cur.act, bad);
-    if (!more) break;
+    load_block<U>(G, pos + 2 * kBlock, 0, ng, b2);
+    hits += lds_lookup<B, 4 * U>(tab, L, displaced, b0.x, b0.act, bad);
     pos += kBlock;
-    cur = nxt;
+    if (pos >= ng) break;
+    load_block<U>(G, pos + 2 * kBlock, 0, ng, b0);
+    hits += lds_lookup<B, 4 * U>(tab, L, displaced, b1.x, b1.act, bad);
+    pos += kBlock;
+    if (pos >= ng) break;
+    load_block<U>(G, pos + 2 * kBlock, 0, ng, b1);
+    hits += lds_lookup<B, 4 * U>(tab, L, displaced, b2.x, b2.act, bad);
+    pos += kBlock;
+    if (pos >= ng) break;
   }
   clear_chunk<CPL>(tab, spos);
   return hits;
