@@ -361,16 +361,18 @@ __device__ __forceinline__ uint32_t pair_resident_wave(const int32_t* __restrict
   return hits;
 }
 
-// Staging of the smaller set: groups of 252 elements, each read by one
-// 16-byte load in lanes 0..62 (252 * 4 bytes is a multiple of 16, so every
-// group keeps the set's 16-byte aligned origin), kept in a two-slot ring in
-// LDS behind each wave's table.
-constexpr int32_t kGroupElems = 4 * (kWave - 1);
-constexpr int32_t kRingSlot = 4 * kWave;
-constexpr int32_t kRing = 2 * kRingSlot;  // int32 per wave
+// Staging of the smaller set: groups of 256 elements, each read by one
+// 16-byte load per lane from the set's 16-byte aligned origin, kept in a
+// two-slot ring in LDS behind each wave's table, so element idx of a staged
+// group sits at ring[(idx - origin) % 512].
+constexpr int32_t kGroupElems = 4 * kWave;
+constexpr int32_t kRing = 2 * kGroupElems;  // int32 per wave
 
 // A pair whose smaller set S exceeds the bin's chunk (both sets ascending).
-// S is cut into chunks of c = min(chunk, 252) elements; chunk k's table
+// S is cut into chunks aligned to the staging groups: each chunk is one
+// group (c = 256 elements) when the bin's table holds 256 keys at load <= 1/2,
+// else c = the largest power of two <= the bin's chunk, so no chunk spans two
+// groups; chunk k's table
 // answers for the values up to its last element, so a block element x is
 // probed against the first chunk whose last element is >= x (the last chunk
 // takes everything above).  When a block reaches past the current chunk the
@@ -391,32 +393,32 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
                                                       int32_t chunk, bool& bad) {
   constexpr int32_t kBlock = 4 * U * kWave;
   const int lane = __lane_id();
-  const int32_t c = chunk < kGroupElems ? chunk : kGroupElems;
-  const uint32_t L = (uint32_t)calc_length(c, alpha, B);
-  // S group g covers element indices [s0 + 252g, s0 + 252g + 252)
+  // the bin's table: calculate_length(chunk) buckets (fits the wave's region)
+  const uint32_t L = (uint32_t)calc_length(chunk, alpha, B);
+  int32_t c = kGroupElems;
+  if (L * (uint32_t)B < 2u * kGroupElems)
+    while (c > chunk) c >>= 1;
+  // S group g covers element indices [s0 + 256g, s0 + 256g + 256)
   const int32_t s0 = block_origin(S);
   const int32_t s_last4 = s0 + ((ns - 1 - s0) & ~3);
-  const int32_t ld = 4 * (lane < kWave - 1 ? lane : kWave - 2);
   auto group_load = [&](int32_t g) {
-    const int32_t e = s0 + g * kGroupElems + ld;
+    const int32_t e = s0 + g * kGroupElems + 4 * lane;
     return __builtin_nontemporal_load(reinterpret_cast<const v4i*>(S + (e < s_last4 ? e : s_last4)));
   };
   auto group_store = [&](int32_t g, const v4i& v) {
-    if (lane < kWave - 1) *reinterpret_cast<v4i*>(ring + (g & 1) * kRingSlot + 4 * lane) = v;
+    *reinterpret_cast<v4i*>(ring + (g & 1) * kGroupElems + 4 * lane) = v;
   };
-  auto group_of = [&](int32_t idx) { return (idx - s0) / kGroupElems; };
-  auto ring_at = [&](int32_t idx) {
-    const int32_t g = (idx - s0) / kGroupElems;
-    return ring[(g & 1) * kRingSlot + (idx - s0 - g * kGroupElems)];
-  };
+  auto group_of = [&](int32_t idx) { return (idx - s0) >> 8; };
+  auto ring_at = [&](int32_t idx) { return ring[(idx - s0) & (kRing - 1)]; };
   {
     const v4i g0 = group_load(0), g1 = group_load(1);
     group_store(0, g0);
     group_store(1, g1);
   }
   int32_t sc = 2;  // next group to stage
-  // the current chunk [cb, ce); its table keys' slots in spos
-  int32_t cb = 0, ce = c < ns ? c : ns;
+  // the current chunk [cb, ce) = element indices [s0 + c k, s0 + c (k+1)) of
+  // the set; its table keys' slots in spos
+  int32_t cb = 0, ce = s0 + c < ns ? s0 + c : ns;
   bool last = ce >= ns;
   int32_t spos[4];
   auto build = [&]() {
@@ -469,7 +471,7 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       // the block reaches past this chunk: rebuild from the next one
       clear_chunk<4>(tab, spos);
       cb = ce;
-      ce = ns - cb < c ? ns : cb + c;
+      ce = ns - cb < c ? ns : cb + c;  // (cb is on the chunk grid from here on)
       last = ce >= ns;
       displaced = build();
       top = last ? 0xFFFFFFFFu : (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
