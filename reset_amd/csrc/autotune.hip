@@ -1,11 +1,13 @@
-// autotune.hip -- per-size-bin choice of (load factor, bucket size, chunk,
-// unroll) for the batched pair kernel, by timing candidates on a sample.
+// autotune.hip -- per-size-bin choice of (load factor, bucket size, chunk)
+// for the batched pair kernel, by timing candidates on a sample (the unroll
+// is the caller's, the same in every bin).
 //
 // Candidates are the settings whose LDS table (calculate_length(chunk) x b
 // slots) fits one wave's 2048-slot region.  They are tried in order of the
 // expected LDS reads per unsuccessful probe under uniform hashing,
 // 1 + (occupied fraction of a home bucket's slots) -- a linear-probing
-// occupancy estimate -- and each bin keeps the fastest measured one.
+// occupancy estimate.  Each bin takes the fastest measured one if it beats
+// the bin's current setting by 3%, else keeps that setting.
 #include <algorithm>
 #include <vector>
 
@@ -23,6 +25,8 @@ int pairs_ctl_status_index();
 using namespace rs;
 
 namespace {
+constexpr float kMargin = 0.97f;  // a candidate must be >= 3% faster than the current setting
+
 struct Cand {
   rs_pairs_bin bin;
   double est;
@@ -85,10 +89,13 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
       result = RS_EDEVICE;
       break;
     }
-    float best = 1e30f;
+    float best = 1e30f, current_ms = 1e30f;
     rs_pairs_bin best_bin = opts->bins[bin];
     best_bin.unroll = opts->bins[0].unroll;
-    // the bin's current setting competes too, so tuning never picks a slower one
+    const rs_pairs_bin current = best_bin;
+    // the bin's current setting competes too, so tuning never picks a slower
+    // one; a candidate must beat it by kMargin to replace it, so timing noise
+    // does not flip the choice from run to run
     std::vector<Cand> tries;
     tries.push_back(Cand{best_bin, 0.0});
     tries.insert(tries.end(), cands.begin(), cands.end());
@@ -109,13 +116,14 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (r > 0) ms_min = std::min(ms_min, ms);
       }
+      if (&c == &tries[0]) current_ms = ms_min;
       if (ms_min < best) {
         best = ms_min;
         best_bin = c.bin;
         best_bin.unroll = opts->bins[0].unroll;
       }
     }
-    opts->bins[bin] = best_bin;
+    opts->bins[bin] = best < kMargin * current_ms ? best_bin : current;
   }
   int32_t err = 0;
   (void)hipMemcpy(&err, ctl.as<int32_t>() + pairs_ctl_status_index(), sizeof(err), hipMemcpyDeviceToHost);

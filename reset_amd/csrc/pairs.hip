@@ -294,7 +294,7 @@ __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, b
     pend[n] = act[n] && !hit && v[n][B - 1] != kEmpty;
     any |= pend[n];
   }
-  if (displaced && __any(any)) {
+  if (displaced && __any(any)) {  // (uniform test first: pend is dead code otherwise)
 #pragma unroll
     for (int n = 0; n < N; ++n) {
       bool p = pend[n];
@@ -401,9 +401,11 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
   // S group g covers element indices [s0 + 256g, s0 + 256g + 256)
   const int32_t s0 = block_origin(S);
   const int32_t s_last4 = s0 + ((ns - 1 - s0) & ~3);
+  // (a plain load: a group is read again each block until a ring slot frees
+  // up, and those reads should hit L2)
   auto group_load = [&](int32_t g) {
     const int32_t e = s0 + g * kGroupElems + 4 * lane;
-    return __builtin_nontemporal_load(reinterpret_cast<const v4i*>(S + (e < s_last4 ? e : s_last4)));
+    return *reinterpret_cast<const v4i*>(S + (e < s_last4 ? e : s_last4));
   };
   auto group_store = [&](int32_t g, const v4i& v) {
     *reinterpret_cast<v4i*>(ring + (g & 1) * kGroupElems + 4 * lane) = v;
