@@ -286,7 +286,9 @@ typedef struct rs_pairs_opts {
   int32_t num_bins;                 /* bins by floor(log2(|A|+|B|)), <= 20   */
   rs_pairs_bin bins[RS_PAIRS_MAX_BINS];
   int32_t waves_per_cu;             /* persistent grid occupancy (4..32)    */
-  int32_t lds_slots_per_wave;       /* int32 LDS slots of a wave's table    */
+  int32_t lds_slots_per_wave;       /* int32 LDS slots of a wave's table;   */
+                                    /* (slots + 512 staging) x 4 B x waves  */
+                                    /* per CU must fit the 160 KiB LDS      */
 } rs_pairs_opts;
 /* Defaults tuned for gfx950 (also the autotuner's starting point). */
 void rs_pairs_default_opts(rs_pairs_opts* o);
