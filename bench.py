@@ -27,6 +27,18 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261203
 
 
+def _cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), for the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def _traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_traffic.json")
@@ -169,6 +181,7 @@ def bench_pairs(args):
             "value": S / ct,
             "unit": "pairs/s",
             "cores": threads,
+            "cpu_model": _cpu_model(),
             "kind": "port",
             "sample": f"first {S} of the {P} pairs (same data), CPU port of set_size/intersection.cu "
                       f"(table of the larger set, smaller probes), OpenMP dynamic schedule",
@@ -261,7 +274,8 @@ def bench_tc(args):
                 break
             rows = min(n, rows * max(2, int(10.0 / max(ct, 1e-3))))
         full = rows >= n
-        out["cpu_baseline"] = {"value": tri_cpu / ct, "unit": "triangles/s", "cores": threads, "kind": "port",
+        out["cpu_baseline"] = {"value": tri_cpu / ct, "unit": "triangles/s", "cores": threads,
+                               "cpu_model": _cpu_model(), "kind": "port",
                                "sample": f"triangles with source vertex in rows [0, {rows}) of {n} "
                                          f"({int(offh[rows])} of {int(offh[-1])} oriented edges), sorted-merge "
                                          "CPU count (the oracle), OpenMP",
@@ -376,7 +390,7 @@ def bench_sss(args):
         ct = time.perf_counter() - t0
         got = api.sss_selfjoin(off[:S + 1].contiguous(), cols, 4, 5)
         out["cpu_baseline"] = {"value": nc / ct, "unit": "candidate pairs verified/s (end to end)",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "cpu_model": _cpu_model(), "kind": "port",
                                "sample": f"self-join of the first {S} sets (prefix filter + merge verification)",
                                "bit_exact_vs_gpu": bool(np.array_equal(ref, got["pairs"].cpu().numpy()))
                                and nc == got["candidates"]}

@@ -4,10 +4,10 @@
 //
 // Candidates are the settings whose LDS table (calculate_length(chunk) x b
 // slots) fits one wave's 2048-slot region.  They are tried in order of the
-// expected LDS reads per unsuccessful probe under uniform hashing,
-// 1 + (occupied fraction of a home bucket's slots) -- a linear-probing
-// occupancy estimate.  Each bin takes the fastest measured one if it beats
-// the bin's current setting by 3%, else keeps that setting.
+// analytic cost model's estimate (the reference's hyperparameter solver,
+// restated in solver.hip), cheapest first.  Each bin takes the fastest
+// measured one if it beats the bin's current setting by 3%, else keeps that
+// setting.
 #include <algorithm>
 #include <vector>
 
@@ -32,24 +32,31 @@ struct Cand {
   double est;
 };
 
-std::vector<Cand> candidates() {
-  const float alphas[] = {0.125f, 0.25f, 0.5f, 0.75f, 0.9f};
-  const int bs[] = {4};
-  const int chunks[] = {127, 255, 511, 1024};
-  std::vector<Cand> out;
-  for (float a : alphas)
-    for (int b : bs)
-      for (int c : chunks) {
-        const int32_t L = calc_length(c, a, b);
-        if (L <= 0 || (int64_t)L * b > 2048) continue;  // default LDS region
-        // fraction of slots in use when the chunk is full
-        const double load = (double)c / ((double)L * b);
-        const double est = 1.0 + load * b / (1.0 + (b - 1) * (1.0 - load)) + 64.0 / c;
-        out.push_back(Cand{rs_pairs_bin{a, b, c, 0}, est});
-      }
-  std::stable_sort(out.begin(), out.end(), [](const Cand& x, const Cand& y) { return x.est < y.est; });
-  if (out.size() > 16) out.resize(16);
-  return out;
+// The prior: the reference's analytic cost model (rs_hyper_cost, the
+// expected probe steps times memory steps per lookup) for a table of `chunk`
+// keys, with gfx950's LDS in place of the reference's GPU parameters -- 64
+// lanes, 128-byte LDS transactions, 4-byte slots.  Computed once.
+const std::vector<Cand>& candidates() {
+  static const std::vector<Cand> cached = [] {
+    const float alphas[] = {0.125f, 0.25f, 0.5f, 0.75f, 0.9f};
+    const int bs[] = {4};
+    const int chunks[] = {127, 255, 511, 1024};
+    std::vector<Cand> out;
+    for (float a : alphas)
+      for (int b : bs)
+        for (int c : chunks) {
+          const int32_t L = calc_length(c, a, b);
+          if (L <= 0 || (int64_t)L * b > 2048) continue;  // default LDS region
+          out.push_back(Cand{rs_pairs_bin{a, b, c, 0}, rs_hyper_cost(c, a, b, 64, 128, 4)});
+        }
+    // cheapest first; among equal costs the larger chunk (fewer rebuilds)
+    std::stable_sort(out.begin(), out.end(), [](const Cand& x, const Cand& y) {
+      return x.est < y.est || (x.est == y.est && x.bin.chunk > y.bin.chunk);
+    });
+    if (out.size() > 16) out.resize(16);
+    return out;
+  }();
+  return cached;
 }
 }  // namespace
 
@@ -79,7 +86,7 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
   hipEvent_t e0, e1;
   RS_HIP(hipEventCreate(&e0));
   RS_HIP(hipEventCreate(&e1));
-  const auto cands = candidates();
+  const auto& cands = candidates();
   rs_status result = RS_OK;
   for (int bin = 0; bin < nb && result == RS_OK; ++bin) {
     if (ids[bin].size() < 16) continue;
