@@ -26,7 +26,6 @@ constexpr int kGroupWaves = kGroupBlock / kWave;
 constexpr int kGroupB = 4;
 constexpr float kGroupAlpha = 0.25f;
 constexpr int64_t kGroupCands = 1024;
-constexpr int kGroupU = 4;  // streamed keys per lane per step
 
 // Largest build set the grouped path takes: calculate_length(2047, 1/4, 4) =
 // 2048 buckets x 4 slots = 32 KiB of LDS per workgroup.
@@ -52,6 +51,28 @@ __global__ void group_flags_kernel(const int32_t* __restrict__ cu, int64_t n,
     flags[i] = i == 0 || cu[i] != cu[i - 1] || i % kGroupCands == 0;
 }
 
+// Streaming the partner sets: each wave takes the unit's candidates
+// c0 + wid, c0 + wid + 4, ... and reads their sets as one stream of 256-token
+// blocks (one 16-byte load per lane from the set's 16-byte aligned origin,
+// clamped to the aligned group holding its last token).  The stream runs
+// through three block buffers in fixed roles, across candidate boundaries,
+// so two blocks are in flight while one is probed; every load is
+// unconditional (past the wave's last candidate it repeats the last address),
+// so the wait before a probe never covers the loads issued after its block.
+// The candidates' set offsets are read into registers when the unit starts.
+constexpr int32_t kStreamBlock = 4 * kWave;
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+static_assert(kGroupCands / kGroupWaves <= 4 * kWave,
+              "a wave's candidates of a unit must fit four registers per lane");
+
+struct PartnerBlock {
+  v4i32 x;
+  int32_t t;    // candidate (wave-local index)
+  int32_t pos;  // first element index of the block (from the set's aligned origin)
+  int32_t nb;   // set size
+  bool valid;
+};
+
 __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
     const int32_t* __restrict__ a_off, const int32_t* __restrict__ a_el,
     const int32_t* __restrict__ b_off, const int32_t* __restrict__ b_el,
@@ -67,6 +88,23 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
     const int64_t c1 = k + 1 < nunits ? unit_start[k + 1] : ncand;
     const int32_t u = cu[c0];
     const int32_t a0 = a_off[u], na = a_off[u + 1] - a0;
+    // this wave's candidates: i_t = c0 + wid + 4t, t < ncw (<= 256); lane l
+    // holds the set offset and size of candidate t = 64r + l in register r
+    const int32_t ncw = c0 + wid < c1 ? (int32_t)((c1 - c0 - wid + kGroupWaves - 1) / kGroupWaves) : 0;
+    int32_t mb0[4], mnb[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t t = r * kWave + lane;
+      const bool ok = t < ncw;
+      const int32_t v = cv[ok ? c0 + wid + (int64_t)kGroupWaves * t : c0];
+      const int32_t b0 = b_off[v];
+      mb0[r] = b0;
+      mnb[r] = ok ? b_off[v + 1] - b0 : 0;
+    }
+    // have them all before the stream starts (their first reads can come
+    // deep inside it, where a wait would also cover the block loads)
+    __asm__ volatile("" : : "v"(mb0[0]), "v"(mb0[1]), "v"(mb0[2]), "v"(mb0[3]), "v"(mnb[0]),
+                     "v"(mnb[1]), "v"(mnb[2]), "v"(mnb[3]));
     const uint32_t L = (uint32_t)calc_length(na, kGroupAlpha, kGroupB);
     const uint32_t nslots = L * kGroupB;  // <= kGroupSlots (build sets are <= kGroupMaxBuild)
     for (uint32_t s = threadIdx.x; s < nslots / 4; s += kGroupBlock)
@@ -83,24 +121,57 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
       bad |= step == nslots;
     }
     __syncthreads();
-    for (int64_t i = c0 + wid; i < c1; i += kGroupWaves) {
-      const int32_t v = cv[i];
-      const int32_t b0 = b_off[v], nb = b_off[v + 1] - b0;
-      const int32_t* __restrict__ G = b_el + b0;
+    if (ncw > 0) {  // wave-uniform
+      auto meta = [&](const int32_t (&m)[4], int32_t t) {
+        const int32_t r = t >> 6;
+        const int32_t v = r == 0 ? m[0] : r == 1 ? m[1] : r == 2 ? m[2] : m[3];
+        return __builtin_amdgcn_readlane(v, t & (kWave - 1));
+      };
+      // load cursor: candidate lt, block position lpos in its set G
+      int32_t lt = 0, lpos = 0, lnb = 0, llast4 = 0;
+      const int32_t* G = nullptr;
+      // (the cursor is wave-uniform; readfirstlane keeps it in scalar
+      // registers, apart from the loads' data)
+      auto enter = [&](int32_t t) {
+        lnb = __builtin_amdgcn_readfirstlane(meta(mnb, t));
+        const int32_t b0 = __builtin_amdgcn_readfirstlane(meta(mb0, t));
+        // an empty set reads (and ignores) the aligned group holding the
+        // build set's first element
+        G = lnb > 0 ? b_el + b0 : a_el + a0;
+        lpos = __builtin_amdgcn_readfirstlane(-(int32_t)((reinterpret_cast<uintptr_t>(G) >> 2) & 3u));
+        llast4 = __builtin_amdgcn_readfirstlane(lnb > 0 ? lpos + ((lnb - 1 - lpos) & ~3) : lpos);
+      };
+      enter(0);
+      auto load_next = [&](PartnerBlock& b) {
+        const int32_t e = lpos + 4 * lane;
+        b.x = __builtin_nontemporal_load(reinterpret_cast<const v4i32*>(G + (e < llast4 ? e : llast4)));
+        b.valid = lt < ncw;
+        b.t = lt;
+        b.pos = lpos;
+        b.nb = lnb;
+        if (lt < ncw) {
+          lpos += kStreamBlock;
+          if (lpos >= lnb && ++lt < ncw) enter(lt);
+        }
+      };
+      // per-candidate counts are kept in registers (candidate t in lane
+      // t % 64 of tot[t / 64]) and written after the stream: a store inside
+      // it would count with the loads and blur the waits before the probes
       uint32_t hits = 0;
-      for (int32_t it = 0; it < nb; it += kWave * kGroupU) {
-        int32_t x[kGroupU];
-        bool act[kGroupU], pend[kGroupU];
-        uint32_t j[kGroupU];
+      int32_t tot[4] = {0, 0, 0, 0};
+      auto probe = [&](const PartnerBlock& b) {
+        // wait for the block on every path (an empty statement that reads
+        // it), so the wait bookkeeping stays exact across the loop
+        __asm__ volatile("" : : "v"(b.x.x));
+        if (!b.valid) return false;
+        const int x[4] = {b.x.x, b.x.y, b.x.z, b.x.w};
+        uint32_t j[4];
+        bool act[4], pend[4];
         bool any = false;
 #pragma unroll
-        for (int q = 0; q < kGroupU; ++q) {
-          const int32_t e = it + q * kWave + lane;
-          act[q] = e < nb;
-          x[q] = G[act[q] ? e : 0];
-        }
-#pragma unroll
-        for (int q = 0; q < kGroupU; ++q) {
+        for (int q = 0; q < 4; ++q) {
+          const int32_t e = b.pos + 4 * lane + q;
+          act[q] = e >= 0 && e < b.nb;
           j[q] = (uint32_t)x[q] & (L - 1u);
           bool hit, full;
           group_bucket(tab, j[q], x[q], hit, full);
@@ -110,7 +181,7 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
         }
         if (__any(any)) {  // rare: walk on past full home buckets
 #pragma unroll
-          for (int q = 0; q < kGroupU; ++q) {
+          for (int q = 0; q < 4; ++q) {
             bool p = pend[q];
             uint32_t jj = j[q];
             for (uint32_t step = 1; step < L && __any(p); ++step) {
@@ -123,9 +194,32 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
             bad |= p;
           }
         }
+        if (b.pos + kStreamBlock >= b.nb) {  // the candidate's last block
+          const int32_t total = (int32_t)wave_sum(hits);
+          const bool mine = lane == (b.t & (kWave - 1));
+          const int32_t r = b.t >> 6;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) tot[q] = q == r && mine ? total : tot[q];
+          hits = 0;
+        }
+        return true;
+      };
+      PartnerBlock p0, p1, p2;
+      load_next(p0);
+      load_next(p1);
+      while (true) {
+        load_next(p2);
+        if (!probe(p0)) break;
+        load_next(p0);
+        if (!probe(p1)) break;
+        load_next(p1);
+        if (!probe(p2)) break;
       }
-      const uint32_t total = wave_sum(hits);
-      if (lane == 0) sizes[i] = (int32_t)total;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int32_t t = r * kWave + lane;
+        if (t < ncw) sizes[c0 + wid + (int64_t)kGroupWaves * t] = tot[r];
+      }
     }
     __syncthreads();
   }
