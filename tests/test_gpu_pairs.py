@@ -75,6 +75,44 @@ def test_batched_chunk_boundaries(api, oracle_lib, sizes):
     plan.close()
 
 
+@pytest.mark.parametrize("b", [4, 2, 1])
+def test_colliding_keys_walk_past_full_buckets(api, oracle_lib, b):
+    """Every element a multiple of 4096: in any table of <= 4096 buckets all
+    keys share home bucket 0, so inserts and lookups walk a chain of full
+    buckets (the displaced path), in resident and chunked pairs alike; a
+    second family mixes such keys with spread ones."""
+    rng = np.random.default_rng(40 + b)
+    sizes = [(40, 3000), (255, 255), (256, 256), (1000, 2000), (2000, 2000), (700, 3000)]
+    a_parts, b_parts, ks = [], [], []
+    for i, (na, nb) in enumerate(sizes * 2):
+        if i < len(sizes):
+            universe = np.arange(4096, dtype=np.int64) * 4096
+        else:  # half colliding, half anywhere else
+            universe = np.unique(np.concatenate([np.arange(2048, dtype=np.int64) * 4096,
+                                                 rng.choice(1 << 24, 6000, replace=False)]))
+        k = min(na, nb) // 2
+        pool = rng.choice(universe, size=na + nb - k, replace=False)
+        a_parts.append(np.sort(pool[:na]))
+        b_parts.append(np.sort(np.concatenate([pool[na - k:na], pool[na:]])))
+        ks.append(k)
+    a_off = np.concatenate([[0], np.cumsum([len(x) for x in a_parts])]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum([len(x) for x in b_parts])]).astype(np.int64)
+    A, B = np.concatenate(a_parts), np.concatenate(b_parts)
+    want = oracle_lib.pairs_hash(a_off, A, b_off, B)
+    assert want.tolist() == ks
+    opts = api.default_pairs_opts()
+    for i in range(opts.num_bins):
+        opts.bins[i].bucket_size = b
+    P = len(ks)
+    plan = api.PairsPlan(P, opts)
+    got = plan.run(*_dev(a_off, A, b_off, B)).cpu().numpy()
+    plan.status()
+    assert got.tolist() == ks
+    got = plan.run(*_dev(b_off, B, a_off, A)).cpu().numpy()
+    assert got.tolist() == ks
+    plan.close()
+
+
 def test_batched_config2_shape_sample(api, oracle_lib):
     """A 20k-pair sample of the 1M-pair workload (log-uniform [16, 65536],
     overlap U[0, 0.5]): batched == plain == k == oracle."""
