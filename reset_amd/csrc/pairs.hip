@@ -86,13 +86,12 @@ struct PairSrcIdx {
 
 // ---------------------------------------------------------------- plan
 // Layout of the plan's control block (int32): [0..kMaxBins) histogram,
-// [kMaxBins..2kMaxBins) cursors, [2kMaxBins] work counter, [2kMaxBins+1]
-// error bits.  Reset by a kernel each launch (no memset inside graphs).
+// [kMaxBins..2kMaxBins) cursors, [2kMaxBins] error bits.  Reset by a kernel
+// each launch (no memset inside graphs).
 constexpr int kCtlHist = 0;
 constexpr int kCtlCursor = RS_PAIRS_MAX_BINS;
-constexpr int kCtlCounter = 2 * RS_PAIRS_MAX_BINS;
-constexpr int kCtlStatus = 2 * RS_PAIRS_MAX_BINS + 1;
-constexpr int kCtlWords = 2 * RS_PAIRS_MAX_BINS + 2;
+constexpr int kCtlStatus = 2 * RS_PAIRS_MAX_BINS;
+constexpr int kCtlWords = 2 * RS_PAIRS_MAX_BINS + 1;
 
 __global__ void plan_reset(int32_t* ctl) {
   if (threadIdx.x < kCtlWords - 1) ctl[threadIdx.x] = 0;  // keeps the error word

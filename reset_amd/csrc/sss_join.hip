@@ -40,8 +40,6 @@ struct rs_sss_result {
 
 namespace rs {
 
-constexpr int kJoinBlock = 256;
-
 __global__ void max_token_kernel(const int32_t* __restrict__ x, int64_t n, int32_t* out,
                                  int32_t* err) {
   int32_t m = -1;
