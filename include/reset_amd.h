@@ -368,6 +368,66 @@ double rs_hyper_cost(int64_t arr_size, double alpha, int32_t b, int32_t W, int32
 rs_status rs_hyper_solve(int64_t arr_size, int32_t W, int32_t S_trans, int32_t S_slot,
                          double* h_alpha, int32_t* h_b, double* h_cost);
 
+/* -------------------------------------------------------- competitor tables */
+/* The per-set tables the reference's baseline directory compares RESET
+ * against, one flat region per set of the CSR they are built from (which
+ * the handle borrows):
+ *  - RS_COMP_LINEAR: cuCollections static_set with linear probing, 1-slot
+ *    buckets and MurmurHash3 fmix32 (baseline/common/cuco.cuh:15-96,
+ *    cuco.cu:35-80), capacity the next power of two of degree/alpha;
+ *  - RS_COMP_CUCKOO: FlatCuckooHash (baseline/common/cuckoo_hash.cuh:
+ *    37-135, cuckoo_hash.cu:40-228): two hashes, <= 6000 evictions, a
+ *    16-entry stash per set;
+ *  - RS_COMP_HOPSCOTCH: HopscotchHash (hopscotch_hash.cuh:35-122,
+ *    hopscotch_hash.cu:27-236): H = 32 neighbourhoods, load factor capped
+ *    at 0.1;
+ *  - RS_COMP_ROARING: RoaringBitmap (roaring_bitmap.cuh:15-113,
+ *    roaring_bitmap.cu:16-139): ids remapped to their rank, sorted 64-bit
+ *    words per set (rows must be ascending; load_factor unused).
+ * Elements must be >= 0 and != INT32_MAX (the EMPTY key).  A key that a
+ * table cannot place (linear table full, cuckoo stash full, hopscotch
+ * displacement failure) is dropped as in the reference and counted. */
+#define RS_COMP_LINEAR 0
+#define RS_COMP_CUCKOO 1
+#define RS_COMP_HOPSCOTCH 2
+#define RS_COMP_ROARING 3
+typedef struct rs_comp rs_comp;
+rs_status rs_comp_build(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
+                        int32_t kind, float load_factor, rs_stream stream, rs_comp** out);
+/* Slots (words for roaring) and the number of dropped keys. */
+rs_status rs_comp_info(const rs_comp* c, int64_t* h_slots, int64_t* h_failed);
+/* The table's arrays, for inspection (0 bytes where a kind has no such
+ * array): region offsets (int64 [n+1]), slots (int32), stash counts (int32
+ * [n]) and stash (int32 [16 n]), hopscotch maps (uint32 per slot), roaring
+ * remap (int32, -1 for ids outside the collection), node starts (int32
+ * [n+1]), word indices (int32) and words (uint64). */
+#define RS_COMP_ARR_OFFSETS 0
+#define RS_COMP_ARR_SLOTS 1
+#define RS_COMP_ARR_STASH_COUNT 2
+#define RS_COMP_ARR_STASH 3
+#define RS_COMP_ARR_BITMAP 4
+#define RS_COMP_ARR_REMAP 5
+#define RS_COMP_ARR_NODE_START 6
+#define RS_COMP_ARR_WORD_INDEX 7
+#define RS_COMP_ARR_WORD_DATA 8
+rs_status rs_comp_array_bytes(const rs_comp* c, int32_t array, int64_t* h_bytes);
+rs_status rs_comp_copy_out(const rs_comp* c, int32_t array, void* h_dst, int64_t bytes);
+/* Triangle counting over the CSR the tables were built from: for every
+ * entry (u, v), |N(s) n N(g)| probed through g's table with s the endpoint
+ * of smaller degree (baseline/tc/tc.cu:35-108, tc_cuco.cu:19-100; the w > v
+ * filter of tc_cuco.cu:81-86 is left out, so degree-oriented CSR counts
+ * too).  Synchronizes the stream. */
+rs_status rs_comp_tc_count(rs_comp* c, uint64_t* h_count, rs_stream stream);
+/* SSS verification of candidate pairs (u_k, v_k) of the table's sets
+ * (sss/sss_cuckoo.cu:19-107, sss_hopscotch.cu, sss_roaring.cu, sss_cuco.cu):
+ * sizes r and flags q*r >= p*(|u|+|v|-r).  Enqueue only. */
+rs_status rs_comp_sss_verify_async(rs_comp* c, const int32_t* d_u, const int32_t* d_v,
+                                   int64_t num_pairs, int32_t p, int32_t q, int32_t* d_flags,
+                                   int32_t* d_sizes, rs_stream stream);
+/* Bad candidate ids from earlier calls (RS_EINVAL); synchronizes. */
+rs_status rs_comp_status(rs_comp* c, rs_stream stream);
+void rs_comp_free(rs_comp* c);
+
 /* --------------------------------------------------------------- generators */
 /* Synthetic inputs (not on the measured path): counter-based, seeded,
  * reproducible.  Pair sets: A_i = sorted(pi_i[0 .. |A_i|)),

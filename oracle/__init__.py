@@ -336,3 +336,153 @@ def ir_merge(idx_off, docs, query, qoff, nthreads=0):
 def ir_result_lists(res, cnt, roff):
     """Per-query result lists as Python lists of ints."""
     return [res[roff[i]:roff[i] + cnt[i]].tolist() for i in range(len(cnt))]
+
+
+# ------------------------------------------------------- competitor tables
+# baseline/common/{cuco,cuckoo_hash,hopscotch_hash,roaring_bitmap}.cu(h),
+# restated in competitors.c.
+COMP_KINDS = {"linear": 0, "cuckoo": 1, "hopscotch": 2, "roaring": 3}
+COMP_EMPTY = 0x7FFFFFFF
+CUCKOO_STASH = 16
+HOP_H = 32
+
+
+class _OrcComp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("toff", C.c_void_p), ("slots", C.c_void_p),
+                ("stash_count", C.c_void_p), ("stash", C.c_void_p), ("bitmap", C.c_void_p),
+                ("remap", C.c_void_p), ("remap_len", C.c_int64), ("node_start", C.c_void_p),
+                ("word_index", C.c_void_p), ("word_data", C.c_void_p)]
+
+
+def _declare_comp(L):
+    if getattr(L, "_comp_declared", False):
+        return
+    L.orc_fmix32.restype = C.c_uint32
+    L.orc_fmix32.argtypes = [C.c_uint32]
+    for g in ("orc_comp_linear_geometry", "orc_comp_cuckoo_geometry", "orc_comp_hopscotch_geometry"):
+        getattr(L, g).restype = _i64
+        getattr(L, g).argtypes = [_p, _i64, _f32, _p]
+    L.orc_comp_linear_build.restype = _i64
+    L.orc_comp_linear_build.argtypes = [_p, _p, _i64, _p, _p]
+    L.orc_comp_linear_check.restype = _i64
+    L.orc_comp_linear_check.argtypes = [_p, _p, _i64, _p, _p]
+    L.orc_comp_cuckoo_build.restype = _i64
+    L.orc_comp_cuckoo_build.argtypes = [_p, _p, _i64, _p, _p, _p, _p]
+    L.orc_comp_hopscotch_build.restype = _i64
+    L.orc_comp_hopscotch_build.argtypes = [_p, _p, _i64, _p, _p, _p]
+    L.orc_comp_roaring_remap.restype = _i64
+    L.orc_comp_roaring_remap.argtypes = [_p, _p, _i64, _p, _i64]
+    L.orc_comp_roaring_words.restype = _i64
+    L.orc_comp_roaring_words.argtypes = [_p, _p, _i64, _p]
+    L.orc_comp_roaring_build.restype = _i64
+    L.orc_comp_roaring_build.argtypes = [_p, _p, _i64, _p, _p, _p, _p]
+    L.orc_comp_contains.restype = C.c_int
+    L.orc_comp_contains.argtypes = [C.POINTER(_OrcComp), _i32, _i32]
+    L.orc_comp_tc.restype = C.c_uint64
+    L.orc_comp_tc.argtypes = [C.POINTER(_OrcComp), _p, _p, _i64]
+    L.orc_comp_sss.restype = None
+    L.orc_comp_sss.argtypes = [C.POINTER(_OrcComp), _p, _p, _p, _p, _i64, _i32, _i32, _p, _p]
+    L._comp_declared = True
+
+
+class CompTable:
+    """One competitor table family built on the CPU: ``arrays`` holds the
+    kind's arrays by name (toff/slots, stash_count/stash, bitmap, remap/
+    node_start/word_index/word_data) and ``failed`` the lost inserts."""
+
+    def __init__(self, kind: str, arrays: dict, failed: int):
+        self.kind, self.arrays, self.failed = kind, arrays, failed
+
+    def struct(self) -> _OrcComp:
+        a = self.arrays
+        s = _OrcComp()
+        s.kind = COMP_KINDS[self.kind]
+        for name in ("toff", "slots", "stash_count", "stash", "bitmap", "remap", "node_start",
+                     "word_index", "word_data"):
+            if name in a:
+                setattr(s, name, a[name].ctypes.data)
+        s.remap_len = len(a["remap"]) if "remap" in a else 0
+        return s
+
+
+def comp_build(kind: str, off, el, load_factor: float = 0.2) -> CompTable:
+    L = lib()
+    _declare_comp(L)
+    off, el = _c(off, np.int32), _c(el, np.int32)
+    n = len(off) - 1
+    if kind == "roaring":
+        remap_len = max(n, int(el.max()) + 1 if len(el) else 0)
+        remap = np.empty(max(remap_len, 1), np.int32)
+        if L.orc_comp_roaring_remap(_ptr(off), _ptr(el), n, _ptr(remap), remap_len) < 0:
+            raise OracleError("roaring: element out of range")
+        remap = remap[:remap_len]
+        W = int(L.orc_comp_roaring_words(_ptr(off), _ptr(el), n, _ptr(remap)))
+        node_start = np.empty(n + 1, np.int32)
+        word_index = np.empty(max(W, 1), np.int32)
+        word_data = np.empty(max(W, 1), np.uint64)
+        L.orc_comp_roaring_build(_ptr(off), _ptr(el), n, _ptr(remap), _ptr(node_start),
+                                 _ptr(word_index), _ptr(word_data))
+        return CompTable(kind, dict(remap=remap, node_start=node_start, word_index=word_index[:W],
+                                    word_data=word_data[:W]), 0)
+    toff = np.empty(n + 1, np.int64)
+    geo = {"linear": L.orc_comp_linear_geometry, "cuckoo": L.orc_comp_cuckoo_geometry,
+           "hopscotch": L.orc_comp_hopscotch_geometry}[kind]
+    total = int(geo(_ptr(off), n, float(load_factor), _ptr(toff)))
+    slots = np.empty(max(total, 1), np.int32)
+    if kind == "linear":
+        failed = L.orc_comp_linear_build(_ptr(off), _ptr(el), n, _ptr(toff), _ptr(slots))
+        return CompTable(kind, dict(toff=toff, slots=slots[:total]), int(failed))
+    if kind == "cuckoo":
+        stash_count = np.empty(max(n, 1), np.int32)
+        stash = np.empty(max(n * CUCKOO_STASH, 1), np.int32)
+        failed = L.orc_comp_cuckoo_build(_ptr(off), _ptr(el), n, _ptr(toff), _ptr(slots),
+                                         _ptr(stash_count), _ptr(stash))
+        return CompTable(kind, dict(toff=toff, slots=slots[:total], stash_count=stash_count[:n],
+                                    stash=stash[:n * CUCKOO_STASH]), int(failed))
+    bitmap = np.empty(max(total, 1), np.uint32)
+    failed = L.orc_comp_hopscotch_build(_ptr(off), _ptr(el), n, _ptr(toff), _ptr(slots), _ptr(bitmap))
+    return CompTable(kind, dict(toff=toff, slots=slots[:total], bitmap=bitmap[:total]), int(failed))
+
+
+def fmix32(x: int) -> int:
+    L = lib()
+    _declare_comp(L)
+    return int(L.orc_fmix32(x & 0xFFFFFFFF))
+
+
+def comp_linear_check(off, el, toff, slots) -> int:
+    """Violations of the linear-probing invariants (0 = a valid table)."""
+    L = lib()
+    _declare_comp(L)
+    off, el = _c(off, np.int32), _c(el, np.int32)
+    toff, slots = _c(toff, np.int64), _c(slots, np.int32)
+    return int(L.orc_comp_linear_check(_ptr(off), _ptr(el), len(off) - 1, _ptr(toff), _ptr(slots)))
+
+
+def comp_contains(t: CompTable, s: int, key: int) -> bool:
+    L = lib()
+    _declare_comp(L)
+    st = t.struct()
+    return bool(L.orc_comp_contains(C.byref(st), int(s), int(key)))
+
+
+def comp_tc(t: CompTable, off, cols) -> int:
+    """Sum over CSR entries (u, v) of |N(s) n N(g)| probed through g's table."""
+    L = lib()
+    _declare_comp(L)
+    off, cols = _c(off, np.int32), _c(cols, np.int32)
+    st = t.struct()
+    return int(L.orc_comp_tc(C.byref(st), _ptr(off), _ptr(cols), len(off) - 1))
+
+
+def comp_sss(t: CompTable, off, el, cand_u, cand_v, p, q):
+    L = lib()
+    _declare_comp(L)
+    off, el = _c(off, np.int32), _c(el, np.int32)
+    cu, cv = _c(cand_u, np.int32), _c(cand_v, np.int32)
+    P = len(cu)
+    flags, sizes = np.empty(P, np.int32), np.empty(P, np.int32)
+    st = t.struct()
+    L.orc_comp_sss(C.byref(st), _ptr(off), _ptr(el), _ptr(cu), _ptr(cv), P, int(p), int(q),
+                   _ptr(flags), _ptr(sizes))
+    return flags, sizes

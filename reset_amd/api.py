@@ -231,6 +231,70 @@ def _sss_result(h, device):
         load().rs_sss_result_free(h)
 
 
+class CompetitorTables:
+    """The baseline directory's competitor tables over one CSR family
+    (baseline/common/{cuco,cuckoo_hash,hopscotch_hash,roaring_bitmap}): kind
+    is "linear" (cuCollections linear probing), "cuckoo", "hopscotch" or
+    "roaring".  The CSR tensors must outlive the object (it borrows them)."""
+
+    _ARRAYS = {"offsets": (0, torch.int64), "slots": (1, torch.int32), "stash_count": (2, torch.int32),
+               "stash": (3, torch.int32), "bitmap": (4, torch.int32), "remap": (5, torch.int32),
+               "node_start": (6, torch.int32), "word_index": (7, torch.int32),
+               "word_data": (8, torch.int64)}
+
+    def __init__(self, offsets, elements, kind="cuckoo", load_factor=0.2, stream=None):
+        _need_cuda(offsets, elements)
+        self.offsets, self.elements = offsets, elements
+        self.kind = kind
+        self.n = offsets.numel() - 1
+        h = C.c_void_p()
+        check(load().rs_comp_build(ptr(offsets), self.n, ptr(elements), _L.COMP_KINDS[kind],
+                                   float(load_factor), stream_handle(stream), C.byref(h)))
+        self._h = h
+        slots, failed = C.c_int64(0), C.c_int64(0)
+        check(load().rs_comp_info(self._h, C.byref(slots), C.byref(failed)))
+        self.slots, self.failed = int(slots.value), int(failed.value)
+
+    def arrays(self) -> dict:
+        """The table's arrays as host tensors (names as in the C header)."""
+        out = {}
+        for name, (code, dt) in self._ARRAYS.items():
+            nb = C.c_int64(0)
+            check(load().rs_comp_array_bytes(self._h, code, C.byref(nb)))
+            if nb.value == 0:
+                continue
+            t = torch.empty(nb.value // torch.empty(0, dtype=dt).element_size(), dtype=dt)
+            check(load().rs_comp_copy_out(self._h, code, C.c_void_p(t.data_ptr()), nb.value))
+            out[name] = t
+        return out
+
+    def tc_count(self, stream=None) -> int:
+        c = C.c_uint64(0)
+        check(load().rs_comp_tc_count(self._h, C.byref(c), stream_handle(stream)))
+        return int(c.value)
+
+    def sss_verify(self, cand_u, cand_v, p=4, q=5, stream=None):
+        _need_cuda(cand_u, cand_v)
+        P = cand_u.numel()
+        flags = torch.empty(max(P, 1), dtype=torch.int32, device=cand_u.device)[:P]
+        sizes = torch.empty(max(P, 1), dtype=torch.int32, device=cand_u.device)[:P]
+        check(load().rs_comp_sss_verify_async(self._h, ptr(cand_u), ptr(cand_v), P, int(p), int(q),
+                                              ptr(flags), ptr(sizes), stream_handle(stream)))
+        check(load().rs_comp_status(self._h, stream_handle(stream)))
+        return flags, sizes
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().rs_comp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, stream=None):
     """Prefix-filter self-join at Jaccard >= p/q with hash verification.
     -> dict(pairs: int64 tensor of i<<32|j ascending (device), candidates,

@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["core.hip", "tables.hip", "tc.hip", "pairs.hip", "sss.hip", "sss_join.hip", "sss_group.hip", "ir.hip",
-           "gen.hip", "autotune.hip", "solver.hip"]
+           "gen.hip", "autotune.hip", "solver.hip", "competitors.hip"]
 TOOLS = ["tc", "sss", "ir", "intersection"]
 
 

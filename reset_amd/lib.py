@@ -110,7 +110,16 @@ SIGNATURES = {
     "rs_hyper_solve": (_i32, [_i64, _i32, _i32, _i32, _p, _p, _p]),
     "rs_gen_pair_sets_async": (_i32, [_p, _p, _p, _i64, _u64, _p, _p, _p]),
     "rs_gen_rmat_async": (_i32, [_i32, _i64, _u32, _u32, _u32, _u64, _p, _p]),
+    "rs_comp_build": (_i32, [_p, _i64, _p, _i32, _f32, _p, _pp]),
+    "rs_comp_info": (_i32, [_p, C.POINTER(_i64), C.POINTER(_i64)]),
+    "rs_comp_array_bytes": (_i32, [_p, _i32, C.POINTER(_i64)]),
+    "rs_comp_copy_out": (_i32, [_p, _i32, _p, _i64]),
+    "rs_comp_tc_count": (_i32, [_p, C.POINTER(_u64), _p]),
+    "rs_comp_sss_verify_async": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p, _p, _p]),
+    "rs_comp_status": (_i32, [_p, _p]),
+    "rs_comp_free": (None, [_p]),
 }
+COMP_KINDS = {"linear": 0, "cuckoo": 1, "hopscotch": 2, "roaring": 3}
 
 _lib = None
 
