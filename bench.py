@@ -8,6 +8,7 @@ persistent size-binned batched launch per step.  Prints ONE JSON line.
 
     python bench.py [--steps K] [--warmup W]            # pairs (default)
     python bench.py --workload tc --scale 24            # R-MAT triangle counting
+    python bench.py --workload compare                  # RESET vs the competitor tables
 
 A step = plan (size binning) + the batched intersection kernel over all pairs,
 inputs resident in HBM.  `roofline.achieved` = algorithmic bytes of one launch
@@ -538,12 +539,103 @@ def _mixed_serving(args, t, queries, resh, cnth, roffh):
             "parity": {"ir_served_equals_batch": bool(ok_ir), "sss_served_equals_batch": bool(ok_s)}}
 
 
+def _timed(fn, steps, warmup):
+    """Average ms of fn() over `steps` calls after `warmup` calls (events on
+    the current stream; fn enqueues or synchronizes itself)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    _sync_all()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        fn()
+    e1.record(st)
+    _sync_all()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_compare(args):
+    """RESET's tables against the baseline directory's competitor tables
+    (linear probing, cuckoo, hopscotch, roaring) on the paper's two uses:
+    TC over a degree-oriented R-MAT CSR and SSS over the edges of a symmetric
+    R-MAT CSR at threshold 1/4 (baseline/tc/main.cu, baseline/sss/main.cu).
+    Not the headline metric: a comparison line."""
+    import torch
+
+    from reset_amd import api, inputs
+
+    kinds = ["linear", "cuckoo", "hopscotch", "roaring"]
+    steps, warm = max(1, args.steps // 2), 1
+    # ---- TC
+    off, cols = inputs.rmat_tc_csr(args.scale, 16, seed=SEED + 3)
+    E = int(cols.numel())
+    t = api.HashTables(off, cols, 0.2, 5, 3)
+    keys = t.probe_order(1)
+    out_t = torch.zeros(1, dtype=torch.int64, device="cuda")
+    tc = {}
+    for name, layout, pk, plain in (("reset_tiled_hierarchical", 1, keys, False),
+                                    ("reset_plain_hierarchical", 1, keys, True),
+                                    ("reset_plain_normal", 0, None, True)):
+        ms = _timed(lambda: t.tc_count_async(out_t, layout, pk, plain), steps, warm)
+        tc[name] = {"count_ms": ms, "triangles": int(out_t.item())}
+    t.close()
+    for k in kinds:
+        _sync_all()
+        t0 = time.perf_counter()
+        c = api.CompetitorTables(off, cols, k, 0.2)
+        _sync_all()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        res = {}
+        ms = _timed(lambda: res.__setitem__("n", c.tc_count()), steps, warm)
+        tc[k] = {"count_ms": ms, "build_ms": build_ms, "triangles": res["n"], "slots": c.slots,
+                 "lost_keys": c.failed}
+        c.close()
+    # ---- SSS (reference form: candidates = the symmetric CSR's edges)
+    lo, hi, n = inputs.rmat_undirected(args.scale - 2, 16, seed=SEED + 4)
+    soff, scols = inputs.symmetric_csr(lo, hi, n)
+    rows = api.expand_rows(soff)
+    P = int(scols.numel())
+    t = api.HashTables(soff, scols, 0.2, 5, 3)
+    sss = {}
+    res = {}
+    ms = _timed(lambda: res.__setitem__("v", t.sss_verify(rows, scols, 1, 4)), steps, warm)
+    flags0, sizes0 = res["v"]
+    sss["reset_normal"] = {"verify_ms": ms, "similar": int(flags0.sum().item())}
+    t.close()
+    for k in kinds:
+        c = api.CompetitorTables(soff, scols, k, 0.2)
+        ms = _timed(lambda: res.__setitem__("v", c.sss_verify(rows, scols, 1, 4)), steps, warm)
+        f, sz = res["v"]
+        sss[k] = {"verify_ms": ms, "similar": int(f.sum().item()), "lost_keys": c.failed,
+                  "sizes_equal_reset": bool(torch.equal(sz, sizes0)) if c.failed == 0 else None}
+        c.close()
+    tri = {v["triangles"] for v in tc.values() if v.get("lost_keys", 0) == 0}
+    best = tc["reset_tiled_hierarchical"]
+    return {
+        "metric": "triangles/s (1 GPU), RESET tables beside the baseline directory's competitor tables",
+        "value": best["triangles"] / (best["count_ms"] / 1000.0),
+        "unit": "triangles/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": best["count_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic R-MAT (a=.57 b=.19 c=.19, seeded, labels scrambled)",
+        "config": {"workload": f"TC on R-MAT scale {args.scale} (degree-oriented CSR, {E} edges); SSS on the "
+                               f"{P} edges of a symmetric R-MAT scale {args.scale - 2} CSR at J >= 1/4",
+                   "load_factor": 0.2, "bucket_size": 5, "parallelism": "single GPU"},
+        "tc": tc, "sss": sss,
+        "parity": {"tc_counts_agree": len(tri) == 1,
+                   "sss_counts_agree": len({v["similar"] for v in sss.values() if v.get("lost_keys", 0) == 0}) == 1},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pairs", choices=["pairs", "tc", "sss", "ir"])
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "tc", "sss", "ir", "compare"])
     ap.add_argument("--sets", type=int, default=1_000_000)
     ap.add_argument("--queries", type=int, default=20_000)
     ap.add_argument("--qps", type=float, default=2000.0)
@@ -556,7 +648,8 @@ def main(argv=None):
     ap.add_argument("--sss-qps", type=float, default=500.0)
     ap.add_argument("--sss-sets", type=int, default=200_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=None,
+                    help="R-MAT scale: 24 for tc, 20 for compare (whose SSS part uses scale - 2)")
     ap.add_argument("--cpu-sample", type=int, default=50_000,
                     help="pairs (pairs workload) or sets (sss workload) in the CPU sample")
     ap.add_argument("--plain-sample", type=int, default=20_000)
@@ -588,7 +681,10 @@ def main(argv=None):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    out = {"pairs": bench_pairs, "tc": bench_tc, "sss": bench_sss, "ir": bench_ir}[args.workload](args)
+    if args.scale is None:
+        args.scale = 20 if args.workload == "compare" else 24
+    out = {"pairs": bench_pairs, "tc": bench_tc, "sss": bench_sss, "ir": bench_ir,
+           "compare": bench_compare}[args.workload](args)
     print(json.dumps(out), flush=True)
     return 0
 
