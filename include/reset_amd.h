@@ -424,7 +424,16 @@ rs_status rs_comp_tc_count(rs_comp* c, uint64_t* h_count, rs_stream stream);
 rs_status rs_comp_sss_verify_async(rs_comp* c, const int32_t* d_u, const int32_t* d_v,
                                    int64_t num_pairs, int32_t p, int32_t q, int32_t* d_flags,
                                    int32_t* d_sizes, rs_stream stream);
-/* Bad candidate ids from earlier calls (RS_EINVAL); synchronizes. */
+/* IR conjunctive queries (ir/ir_cuco.cu:11-110) over tables built on an
+ * inverted index (sets = terms, elements = doc ids): query q's terms are
+ * d_query[d_query_offsets[q] .. d_query_offsets[q+1]), term0 first; its
+ * survivors, in term0's order, go to d_result[d_result_offsets[q] ..) (a
+ * region of |list(term0)| entries) and their number to d_result_count[q].
+ * Enqueue only. */
+rs_status rs_comp_ir_query_async(rs_comp* c, const int32_t* d_query, const int32_t* d_query_offsets,
+                                 int64_t num_queries, const int64_t* d_result_offsets,
+                                 int32_t* d_result, int32_t* d_result_count, rs_stream stream);
+/* Bad candidate ids or query terms from earlier calls (RS_EINVAL); synchronizes. */
 rs_status rs_comp_status(rs_comp* c, rs_stream stream);
 void rs_comp_free(rs_comp* c);
 

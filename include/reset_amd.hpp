@@ -464,6 +464,16 @@ class IRBaseline : public HashTableBuilder {
     return out;
   }
   int get_query_num() const { return query_num_; }
+  // Device buffers (baseline/ir/ir.cuh:55-62), valid after
+  // allocate_result_buffers; the reference's work counter is not needed.
+  int* get_d_query() const { return d_query_.data(); }
+  int* get_d_query_offsets() const { return d_query_offsets_.data(); }
+  int* get_d_result() const { return d_result_.data(); }
+  long long* get_d_result_offsets() const {
+    return reinterpret_cast<long long*>(d_result_offsets_.data());
+  }
+  int* get_d_result_count() const { return d_result_count_.data(); }
+  int* get_d_G_index() const { return nullptr; }
 
  private:
   std::pair<int, float> run(int layout, int bucket_size, bool sorted) {
@@ -491,6 +501,113 @@ class IRBaseline : public HashTableBuilder {
   DeviceArray<int64_t> d_result_offsets_;
   DeviceArray<int> sorted_keys_;
 };
+
+// ------------------------------------------------------------ competitor tables
+// The free functions of baseline/tc/tc_cuco.cuh:1-32, baseline/sss/
+// sss_baselines.h:14-70 and sss_cuco.cuh over the competitor tables
+// (rs_comp_*): the tables are built from the device CSR (which must hold the
+// host CSR's values), then the probe kernel alone is timed.  grid_size,
+// block_size and CHUNK_SIZE are ignored; the threshold is read as its exact
+// fraction; keys a table could not place are reported on stderr, as the
+// reference warns of them.
+namespace detail {
+class CompHandle {
+ public:
+  CompHandle(const int* d_offsets, int num_sets, const int* d_elements, int kind, float load_factor) {
+    check(rs_comp_build(d_offsets, num_sets, d_elements, kind, load_factor, nullptr, &c_),
+          "rs_comp_build");
+    int64_t slots = 0, failed = 0;
+    check(rs_comp_info(c_, &slots, &failed), "rs_comp_info");
+    if (failed) std::fprintf(stderr, "warning: %lld keys could not be inserted\n", (long long)failed);
+  }
+  ~CompHandle() { rs_comp_free(c_); }
+  CompHandle(const CompHandle&) = delete;
+  CompHandle& operator=(const CompHandle&) = delete;
+  rs_comp* get() const { return c_; }
+
+ private:
+  rs_comp* c_ = nullptr;
+};
+
+inline std::pair<int, float> run_sss_kind(int kind, int num_pairs, int num_nodes, int const* d_vertexs,
+                                          int const* d_csr_cols_for_vertexs, int const* d_csr_cols,
+                                          int const* d_csr_offsets, float threshold, float load_factor) {
+  const auto pq = threshold_fraction(threshold);
+  CompHandle c(d_csr_offsets, num_nodes, d_csr_cols, kind, load_factor);
+  DeviceArray<int> flags(num_pairs), sizes(num_pairs);
+  const float ms = time_ms([&] {
+    check(rs_comp_sss_verify_async(c.get(), d_vertexs, d_csr_cols_for_vertexs, num_pairs, pq.first,
+                                   pq.second, flags.data(), sizes.data(), nullptr),
+          "rs_comp_sss_verify_async");
+  });
+  check(rs_comp_status(c.get(), nullptr), "rs_comp_sss_verify_async");
+  int64_t n = 0;
+  for (int f : flags.to_host()) n += f;
+  return {(int)n, ms};
+}
+}  // namespace detail
+
+// tc_cuco.cuh: TC through linear-probing (cuCollections-style) tables.
+inline std::pair<unsigned long long, float> run_tc_cuco(
+    int num_nodes, int num_edges, int const* d_vertexs, int const* d_csr_row,
+    int const* d_csr_cols_for_traversal, std::vector<int> const& csr_row_host,
+    std::vector<int> const& csr_cols_host, int grid_size, int block_size, int CHUNK_SIZE,
+    float load_factor, hipStream_t stream = nullptr) {
+  (void)num_edges, (void)d_vertexs, (void)csr_row_host, (void)csr_cols_host, (void)grid_size,
+      (void)block_size, (void)CHUNK_SIZE, (void)stream;
+  detail::CompHandle c(d_csr_row, num_nodes, d_csr_cols_for_traversal, RS_COMP_LINEAR, load_factor);
+  uint64_t count = 0;
+  check(rs_comp_tc_count(c.get(), &count, nullptr), "rs_comp_tc_count");  // warm-up (row ids)
+  const float ms = time_ms([&] { check(rs_comp_tc_count(c.get(), &count, nullptr), "rs_comp_tc_count"); });
+  return {(unsigned long long)count, ms};
+}
+
+// ir_cuco.cuh: conjunctive queries through linear-probing tables built over
+// the inverted index; returns (total result count, kernel ms).
+inline std::pair<int, float> run_ir_cuco(
+    int inverted_index_num, int query_num, int const* d_inverted_index,
+    int const* d_inverted_index_offsets, int const* d_query, int const* d_query_offsets,
+    int* d_result, long long const* d_result_offsets, int* d_result_count, int* d_G_index,
+    int CHUNK_SIZE, std::vector<int> const& inverted_index_offsets_host,
+    std::vector<int> const& inverted_index_host, int grid_size, int block_size, float load_factor,
+    hipStream_t stream = nullptr) {
+  (void)d_G_index, (void)CHUNK_SIZE, (void)inverted_index_offsets_host, (void)inverted_index_host,
+      (void)grid_size, (void)block_size, (void)stream;
+  static_assert(sizeof(long long) == sizeof(int64_t), "64-bit result offsets");
+  detail::CompHandle c(d_inverted_index_offsets, inverted_index_num, d_inverted_index, RS_COMP_LINEAR,
+                       load_factor);
+  const float ms = time_ms([&] {
+    check(rs_comp_ir_query_async(c.get(), d_query, d_query_offsets, query_num,
+                                 reinterpret_cast<const int64_t*>(d_result_offsets), d_result,
+                                 d_result_count, nullptr),
+          "rs_comp_ir_query_async");
+  });
+  check(rs_comp_status(c.get(), nullptr), "rs_comp_ir_query_async");
+  std::vector<int> counts(query_num);
+  if (query_num)
+    check_hip(hipMemcpy(counts.data(), d_result_count, sizeof(int) * query_num, hipMemcpyDeviceToHost),
+              "hipMemcpy");
+  int64_t total = 0;
+  for (int v : counts) total += v;
+  return {(int)total, ms};
+}
+
+#define RESET_AMD_SSS_COMP(name, kind)                                                             \
+  inline std::pair<int, float> name(                                                               \
+      int num_pairs, int num_nodes, int const* d_vertexs, int const* d_csr_cols_for_vertexs,       \
+      int const* d_csr_cols, int const* d_csr_offsets, std::vector<int> const& csr_offsets_host,   \
+      std::vector<int> const& csr_cols_host, float threshold, int grid_size, int block_size,       \
+      int CHUNK_SIZE, float load_factor = 0.2f, hipStream_t stream = nullptr) {                    \
+    (void)csr_offsets_host, (void)csr_cols_host, (void)grid_size, (void)block_size,                \
+        (void)CHUNK_SIZE, (void)stream;                                                            \
+    return detail::run_sss_kind(kind, num_pairs, num_nodes, d_vertexs, d_csr_cols_for_vertexs,     \
+                                d_csr_cols, d_csr_offsets, threshold, load_factor);                \
+  }
+RESET_AMD_SSS_COMP(run_sss_cuco, RS_COMP_LINEAR)          // sss_cuco.cuh
+RESET_AMD_SSS_COMP(run_sss_cuckoo, RS_COMP_CUCKOO)        // sss_baselines.h:32-38
+RESET_AMD_SSS_COMP(run_sss_hopscotch, RS_COMP_HOPSCOTCH)  // sss_baselines.h:43-49
+RESET_AMD_SSS_COMP(run_sss_roaring, RS_COMP_ROARING)      // sss_baselines.h:54-60
+#undef RESET_AMD_SSS_COMP
 
 // ------------------------------------------------------------------ pairs
 // set_size/intersection.cu:128-181 for one pair, and the batched form for

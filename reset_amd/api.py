@@ -283,6 +283,20 @@ class CompetitorTables:
         check(load().rs_comp_status(self._h, stream_handle(stream)))
         return flags, sizes
 
+    def ir_query(self, query, query_offsets, stream=None):
+        """-> (result int32[roff[Q]], count int32[Q], roff int64[Q+1]); the
+        tables must be built over an inverted index (sets = terms)."""
+        _need_cuda(query, query_offsets)
+        Q = query_offsets.numel() - 1
+        roff = ir_result_offsets(self.offsets, query, query_offsets)
+        total = int(roff[-1].item()) if Q else 0
+        res = torch.full((max(total, 1),), -1, dtype=torch.int32, device=query.device)
+        cnt = torch.empty(max(Q, 1), dtype=torch.int32, device=query.device)
+        check(load().rs_comp_ir_query_async(self._h, ptr(query), ptr(query_offsets), Q, ptr(roff), ptr(res),
+                                            ptr(cnt), stream_handle(stream)))
+        check(load().rs_comp_status(self._h, stream_handle(stream)))
+        return res[:total], cnt[:Q], roff
+
     def close(self):
         if getattr(self, "_h", None):
             load().rs_comp_free(self._h)

@@ -18,7 +18,8 @@
 // offsets overflow past 2^31 slots).  Consumers: TC over the CSR the tables
 // were built from and SSS verification of candidate pairs, a wave per edge
 // or pair with the smaller set's lanes probing the larger set's table
-// (baseline/tc/tc.cu:35-108, tc_cuco.cu, sss/sss_cuckoo.cu:19-107).
+// (baseline/tc/tc.cu:35-108, tc_cuco.cu, sss/sss_cuckoo.cu:19-107), and IR
+// conjunctive queries over tables of an inverted index (ir/ir_cuco.cu).
 #include "common.hpp"
 
 namespace rs {
@@ -446,6 +447,57 @@ __global__ void __launch_bounds__(kCompBlock) comp_sss_kernel(const CompView c, 
   if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
 }
 
+// IR conjunctive queries over tables of an inverted index (ir/ir_cuco.cu:
+// 11-110): a wave per query copies term0's list into its result region and
+// keeps, in order, the documents every later term's table holds (stable
+// in-place compaction: a write never passes the read position).
+template <int KIND>
+__global__ void __launch_bounds__(kCompBlock) comp_ir_kernel(
+    const CompView c, const int32_t* __restrict__ off, const int32_t* __restrict__ docs, int64_t n,
+    const int32_t* __restrict__ query, const int32_t* __restrict__ qoff, int64_t Q,
+    const int64_t* __restrict__ roff, int32_t* result, int32_t* __restrict__ count, int32_t* status) {
+  const int lane = __lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  bool bad = false;
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < Q; q += waves) {
+    const int32_t qs = qoff[q], qe = qoff[q + 1];
+    bool ok = qe > qs;
+    for (int32_t i = qs; i < qe; ++i) ok = ok && query[i] >= 0 && query[i] < n;
+    if (!ok) {  // an empty query, or a term outside the index (flagged)
+      bad |= qe > qs;
+      if (lane == 0) count[q] = 0;
+      continue;
+    }
+    const int32_t t0 = query[qs];
+    const int32_t n0 = off[t0 + 1] - off[t0];
+    const int32_t* src = docs + off[t0];
+    int32_t* dst = result + roff[q];
+    int32_t cur = n0;
+    if (qe - qs == 1)
+      for (int32_t j = lane; j < n0; j += kWave) dst[j] = src[j];
+    for (int32_t i = qs + 1; i < qe && cur > 0; ++i) {
+      const int32_t term = query[i];
+      int32_t w = 0;
+      for (int32_t j0 = 0; j0 < cur; j0 += kWave) {
+        const int32_t j = j0 + lane;
+        int32_t key = 0;
+        bool hit = false;
+        if (j < cur) {
+          key = src[j];
+          hit = comp_contains<KIND>(c, term, key);
+        }
+        const uint64_t m = __ballot(hit);
+        if (hit) dst[w + __popcll(m & lanemask_lt())] = key;
+        w += __popcll(m);
+      }
+      cur = w;
+      src = dst;
+    }
+    if (lane == 0) count[q] = cur;
+  }
+  if (__any(bad) && lane == 0) atomicOr(status, kErrBadInput);
+}
+
 }  // namespace rs
 
 using namespace rs;
@@ -581,6 +633,14 @@ void launch_sss(const rs_comp* c, const int32_t* u, const int32_t* v, int64_t P,
   const int g = grid_for(P * kWave, kCompBlock);
   comp_sss_kernel<KIND><<<g, kCompBlock, 0, st>>>(c->view(), c->d_off, c->d_el, u, v, P, p, q, c->n, flags, sizes,
                                                   c->status());
+}
+
+template <int KIND>
+void launch_ir(const rs_comp* c, const int32_t* query, const int32_t* qoff, int64_t Q, const int64_t* roff,
+               int32_t* result, int32_t* count, hipStream_t st) {
+  const int g = grid_for(Q * kWave, kCompBlock);
+  comp_ir_kernel<KIND><<<g, kCompBlock, 0, st>>>(c->view(), c->d_off, c->d_el, c->n, query, qoff, Q, roff, result,
+                                                 count, c->status());
 }
 
 }  // namespace
@@ -724,6 +784,36 @@ rs_status rs_comp_sss_verify_async(rs_comp* c, const int32_t* d_u, const int32_t
       launch_sss<RS_COMP_HOPSCOTCH>(c, d_u, d_v, num_pairs, p, q, d_flags, d_sizes, st);
       break;
     default: launch_sss<RS_COMP_ROARING>(c, d_u, d_v, num_pairs, p, q, d_flags, d_sizes, st); break;
+  }
+  RS_HIP(hipGetLastError());
+  return RS_OK;
+}
+
+rs_status rs_comp_ir_query_async(rs_comp* c, const int32_t* d_query, const int32_t* d_query_offsets,
+                                 int64_t num_queries, const int64_t* d_result_offsets, int32_t* d_result,
+                                 int32_t* d_result_count, rs_stream stream) {
+  RS_REQUIRE(c != nullptr && num_queries >= 0, "comp: bad arguments");
+  RS_REQUIRE(num_queries == 0 || (d_query_offsets && d_result_offsets && d_result_count),
+             "comp: null query or output buffer");
+  hipStream_t st = as_stream(stream);
+  if (num_queries == 0) return RS_OK;
+  switch (c->kind) {
+    case RS_COMP_LINEAR:
+      launch_ir<RS_COMP_LINEAR>(c, d_query, d_query_offsets, num_queries, d_result_offsets, d_result,
+                                d_result_count, st);
+      break;
+    case RS_COMP_CUCKOO:
+      launch_ir<RS_COMP_CUCKOO>(c, d_query, d_query_offsets, num_queries, d_result_offsets, d_result,
+                                d_result_count, st);
+      break;
+    case RS_COMP_HOPSCOTCH:
+      launch_ir<RS_COMP_HOPSCOTCH>(c, d_query, d_query_offsets, num_queries, d_result_offsets, d_result,
+                                   d_result_count, st);
+      break;
+    default:
+      launch_ir<RS_COMP_ROARING>(c, d_query, d_query_offsets, num_queries, d_result_offsets, d_result,
+                                 d_result_count, st);
+      break;
   }
   RS_HIP(hipGetLastError());
   return RS_OK;

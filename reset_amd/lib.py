@@ -116,6 +116,7 @@ SIGNATURES = {
     "rs_comp_copy_out": (_i32, [_p, _i32, _p, _i64]),
     "rs_comp_tc_count": (_i32, [_p, C.POINTER(_u64), _p]),
     "rs_comp_sss_verify_async": (_i32, [_p, _p, _p, _i64, _i32, _i32, _p, _p, _p]),
+    "rs_comp_ir_query_async": (_i32, [_p, _p, _p, _i64, _p, _p, _p, _p]),
     "rs_comp_status": (_i32, [_p, _p]),
     "rs_comp_free": (None, [_p]),
 }

@@ -95,6 +95,17 @@ def test_cli_argument_and_file_errors(built_tools, tmp_path, tool):
     assert r.returncode == 1 and "Error opening file" in r.stderr
 
 
+@pytest.mark.parametrize("tool,bad", [("tc", "cuckoo"), ("sss", "bogus"), ("ir", "roaring")])
+def test_cli_unknown_method(built_tools, tmp_path, tool, bad):
+    """--method= lists the tool's methods (tc and ir: original, cuco; sss:
+    original, cuco, cuckoo, hopscotch, roaring); others exit 1 before any
+    file or device is touched (baseline/tc/main.cu:55-70)."""
+    exe = os.path.join(built_tools, tool)
+    r = subprocess.run([exe, str(tmp_path), f"--method=original,{bad}"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and f"Unknown method: {bad}" in r.stderr
+
+
 def test_intersection_cli_usage(built_tools):
     r = subprocess.run([os.path.join(built_tools, "intersection"), "5"], capture_output=True, text=True,
                        timeout=60)

@@ -67,6 +67,38 @@ def test_sss_cli_matches_float_rule(tmp_path, oracle_lib):
     assert _num(r"greater than threshold: (\d+)", out) == want
 
 
+def test_cli_competitor_methods_agree(tmp_path, oracle_lib):
+    """--method=original,cuco (tc, ir) and --method=all (sss): RESET's tables
+    and the competitor tables print the same counts (baseline/tc/main.cu,
+    baseline/sss/main.cu, baseline/ir/main.cu)."""
+    from reset_amd import binfmt
+
+    rng = np.random.default_rng(21)
+    n = 2000
+    clique = np.array([(i, j) for i in range(50) for j in range(i + 1, 50)])
+    und = G.undirected(np.concatenate([G.random_graph(n, 30000, rng), clique]), n)
+    off, cols = G.orient(und, n, "id")
+    want = oracle_lib.tc_merge(off, cols)
+    binfmt.write_tc_folder(tmp_path / "tc", off, cols)
+    out = _run(["tc", tmp_path / "tc", "--method=original,cuco"])
+    assert _num(r"Hierarchical hash - Triangles: (\d+)", out) == want
+    assert _num(r"\[cuCollections\] Triangles: (\d+)", out) == want
+    soff, scols = G.symmetric(und, n)
+    binfmt.write_sss_folder(tmp_path / "sss", soff, scols)
+    out = _run(["sss", tmp_path / "sss", "--method=all"])
+    base = _num(r"(?m)^num of pairs whose similarity is greater than threshold: (\d+)", out)
+    for label in ("cuCollections", "cuckoo", "hopscotch", "roaring"):
+        assert _num(r"\[" + label + r"\] num of pairs whose similarity is greater than threshold: (\d+)", out) == base
+    assert "Wrong Answer" not in out
+    idx_off, docs = G.inverted_index(rng, 200, 100000, 8000)
+    q, qoff = G.queries(rng, idx_off, 150, 1, 5)
+    _, m_cnt, _ = oracle_lib.ir_merge(idx_off, docs, q, qoff)
+    binfmt.write_ir_folder(tmp_path / "ir", idx_off, docs, q, qoff)
+    out = _run(["ir", tmp_path / "ir", "--method=original,cuco"])
+    assert _num(r"(?m)^Total result count: (\d+)", out) == int(m_cnt.sum())
+    assert _num(r"\[cuCollections\] Total result count: (\d+)", out) == int(m_cnt.sum())
+
+
 def test_ir_cli_matches_chained_merge(tmp_path, oracle_lib):
     from reset_amd import binfmt
 

@@ -147,6 +147,25 @@ def test_sss_verify_equals_bruteforce(api, oracle_lib, kind):
     t.close()
 
 
+@pytest.mark.parametrize("kind", KINDS)
+def test_ir_queries_equal_chained_merge(api, oracle_lib, kind):
+    """Conjunctive queries through tables of an inverted index (ir_cuco.cu):
+    counts and result lists (term0 order, ascending doc ids) equal the
+    chained merge, single-term and empty queries included."""
+    rng = np.random.default_rng(17)
+    idx_off, docs = G.inverted_index(rng, 300, 200000, 20000)
+    q, qoff = G.queries(rng, idx_off, 400, 1, 6)
+    q = np.concatenate([q, [5]]).astype(np.int32)
+    qoff = np.concatenate([qoff, [qoff[-1], qoff[-1] + 1]]).astype(np.int32)  # an empty query, then {5}
+    m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
+    t = api.CompetitorTables(_cuda(idx_off), _cuda(docs), kind, 0.2)
+    res, cnt, roff = t.ir_query(_cuda(q), _cuda(qoff))
+    assert np.array_equal(cnt.cpu().numpy(), m_cnt)
+    assert oracle_lib.ir_result_lists(res.cpu().numpy(), cnt.cpu().numpy(), roff.cpu().numpy()) == \
+        oracle_lib.ir_result_lists(m_res, m_cnt, m_roff)
+    t.close()
+
+
 def test_input_errors(api):
     from reset_amd.lib import ResetError
 
@@ -160,4 +179,6 @@ def test_input_errors(api):
     t = api.CompetitorTables(_cuda(off), _cuda(el), "cuckoo")
     with pytest.raises(ResetError):
         t.sss_verify(_cuda(np.array([0, 5], np.int32)), _cuda(np.array([1, 0], np.int32)))
+    with pytest.raises(ResetError):  # a query term outside the index
+        t.ir_query(_cuda(np.array([0, 7], np.int32)), _cuda(np.array([0, 2], np.int32)))
     t.close()
