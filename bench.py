@@ -599,11 +599,17 @@ def bench_compare(args):
     rows = api.expand_rows(soff)
     P = int(scols.numel())
     t = api.HashTables(soff, scols, 0.2, 5, 3)
+    skeys = t.probe_order(1)
     sss = {}
     res = {}
     ms = _timed(lambda: res.__setitem__("v", t.sss_verify(rows, scols, 1, 4)), steps, warm)
     flags0, sizes0 = res["v"]
     sss["reset_normal"] = {"verify_ms": ms, "similar": int(flags0.sum().item())}
+    # RESET's method: hierarchical tables, probe keys read in table order
+    ms = _timed(lambda: res.__setitem__("v", t.sss_verify(rows, scols, 1, 4, layout=1, probe_keys=skeys)),
+                steps, warm)
+    sss["reset_hierarchical"] = {"verify_ms": ms, "similar": int(res["v"][0].sum().item()),
+                                 "sizes_equal_reset": bool(torch.equal(res["v"][1], sizes0))}
     t.close()
     for k in kinds:
         c = api.CompetitorTables(soff, scols, k, 0.2)
