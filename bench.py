@@ -559,10 +559,11 @@ def _timed(fn, steps, warmup):
 
 def bench_compare(args):
     """RESET's tables against the baseline directory's competitor tables
-    (linear probing, cuckoo, hopscotch, roaring) on the paper's two uses:
-    TC over a degree-oriented R-MAT CSR and SSS over the edges of a symmetric
-    R-MAT CSR at threshold 1/4 (baseline/tc/main.cu, baseline/sss/main.cu).
-    Not the headline metric: a comparison line."""
+    (linear probing, cuckoo, hopscotch, roaring) on the reference's three
+    comparisons: TC over a degree-oriented R-MAT CSR, SSS over the edges of a
+    symmetric R-MAT CSR at threshold 1/4, and IR conjunctive queries
+    (baseline/{tc,sss,ir}/main.cu).  Not the headline metric: a comparison
+    line."""
     import torch
 
     from reset_amd import api, inputs
@@ -618,6 +619,24 @@ def bench_compare(args):
         sss[k] = {"verify_ms": ms, "similar": int(f.sum().item()), "lost_keys": c.failed,
                   "sizes_equal_reset": bool(torch.equal(sz, sizes0)) if c.failed == 0 else None}
         c.close()
+    # ---- IR (the reference's ir_cuco comparison), a 1M-document index
+    ioff, idocs = inputs.ir_index(num_docs=1_000_000, num_terms=100_000, cap=200_000, seed=SEED + 5)
+    q, qoff = inputs.ir_queries(ioff, 20_000, seed=SEED + 6)
+    qd, qod = torch.from_numpy(q).cuda(), torch.from_numpy(qoff).cuda()
+    t = api.HashTables(ioff, idocs, 0.2, 5, 3)
+    ikeys = t.probe_order(1)
+    ir = {}
+    ms = _timed(lambda: res.__setitem__("v", t.ir_query(qd, qod, 0)), steps, warm)
+    total0 = int(res["v"][1].sum().item())
+    ir["reset_normal"] = {"query_ms": ms, "results": total0}
+    ms = _timed(lambda: res.__setitem__("v", t.ir_query(qd, qod, 1, term0_docs=ikeys)), steps, warm)
+    ir["reset_hierarchical"] = {"query_ms": ms, "results": int(res["v"][1].sum().item())}
+    t.close()
+    for k in kinds:
+        c = api.CompetitorTables(ioff, idocs, k, 0.2)
+        ms = _timed(lambda: res.__setitem__("v", c.ir_query(qd, qod)), steps, warm)
+        ir[k] = {"query_ms": ms, "results": int(res["v"][1].sum().item()), "lost_keys": c.failed}
+        c.close()
     tri = {v["triangles"] for v in tc.values() if v.get("lost_keys", 0) == 0}
     best = tc["reset_tiled_hierarchical"]
     return {
@@ -628,11 +647,13 @@ def bench_compare(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
         "data": "synthetic R-MAT (a=.57 b=.19 c=.19, seeded, labels scrambled)",
         "config": {"workload": f"TC on R-MAT scale {args.scale} (degree-oriented CSR, {E} edges); SSS on the "
-                               f"{P} edges of a symmetric R-MAT scale {args.scale - 2} CSR at J >= 1/4",
+                               f"{P} edges of a symmetric R-MAT scale {args.scale - 2} CSR at J >= 1/4; IR: 20k "
+                               f"queries over a {int(ioff[-1])}-posting index (1M docs, 100k terms)",
                    "load_factor": 0.2, "bucket_size": 5, "parallelism": "single GPU"},
-        "tc": tc, "sss": sss,
+        "tc": tc, "sss": sss, "ir": ir,
         "parity": {"tc_counts_agree": len(tri) == 1,
-                   "sss_counts_agree": len({v["similar"] for v in sss.values() if v.get("lost_keys", 0) == 0}) == 1},
+                   "sss_counts_agree": len({v["similar"] for v in sss.values() if v.get("lost_keys", 0) == 0}) == 1,
+                   "ir_counts_agree": len({v["results"] for v in ir.values() if v.get("lost_keys", 0) == 0}) == 1},
     }
 
 
