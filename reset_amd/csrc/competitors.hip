@@ -51,13 +51,20 @@ __device__ __forceinline__ int32_t std_hash2(int32_t key, int32_t set, int32_t c
 
 // ---------------------------------------------------------------- geometry
 // Region size of set i (cuco.cu:69-80, cuckoo_hash.cu:150-170,
-// hopscotch_hash.cu:183-202).
+// hopscotch_hash.cu:183-202).  A set whose degree / load factor exceeds
+// 2^30 slots is flagged (bad input) and given one slot.
 __global__ void comp_capacity_kernel(const int32_t* __restrict__ off, int64_t n, float lf, int32_t kind,
-                                     int64_t* __restrict__ cap) {
+                                     int64_t* __restrict__ cap, int32_t* status) {
+  bool bad = false;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t d = off[i + 1] - off[i];
     int64_t c = 1;
+    if (!((double)d / (double)(lf < 0.1f || kind != RS_COMP_HOPSCOTCH ? lf : 0.1f) <= 1073741824.0)) {
+      bad = true;
+      cap[i] = 1;
+      continue;
+    }
     if (kind == RS_COMP_LINEAR) {
       const uint64_t req = (uint64_t)((float)d / lf);
       while ((uint64_t)c < req) c <<= 1;
@@ -73,6 +80,7 @@ __global__ void comp_capacity_kernel(const int32_t* __restrict__ off, int64_t n,
     }
     cap[i] = c;
   }
+  if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
 }
 
 template <typename T>
@@ -558,7 +566,8 @@ rs_status build_regions(rs_comp* c, float lf, hipStream_t st) {
   DevBuf cap;
   RS_TRY(cap.alloc(sizeof(int64_t) * (size_t)(c->n + 1)));
   const int g = grid_for(c->n, kCompBlock);
-  comp_capacity_kernel<<<g, kCompBlock, 0, st>>>(c->d_off, c->n, lf, c->kind, cap.as<int64_t>());
+  comp_capacity_kernel<<<g, kCompBlock, 0, st>>>(c->d_off, c->n, lf, c->kind, cap.as<int64_t>(), c->status());
+  RS_TRY(check_status(c, st, "comp build (a set needs more than 2^30 slots at this load factor)"));
   RS_TRY(c->toff.alloc(sizeof(int64_t) * (size_t)(c->n + 1)));
   RS_TRY(exclusive_scan_i64(cap.as<int64_t>(), c->n, c->toff.as<int64_t>(), c->tmp, st));
   RS_HIP(hipMemcpyAsync(&c->slots_total, c->toff.as<int64_t>() + c->n, sizeof(int64_t),

@@ -182,3 +182,7 @@ def test_input_errors(api):
     with pytest.raises(ResetError):  # a query term outside the index
         t.ir_query(_cuda(np.array([0, 7], np.int32)), _cuda(np.array([0, 2], np.int32)))
     t.close()
+    off, el = _csr([list(range(5000))])  # 5000 / 1e-6 slots: beyond 2^30
+    for kind in ("linear", "cuckoo", "hopscotch"):
+        with pytest.raises(ResetError):
+            api.CompetitorTables(_cuda(off), _cuda(el), kind, 1e-6)
