@@ -36,6 +36,8 @@ typedef int32_t rs_status;
 #define RS_EOVERFLOW 3  /* bounded probe or insert hit a full table       */
 #define RS_EDEVICE 4    /* HIP runtime error                              */
 #define RS_EBUSY 5      /* serving queue full; retry after polling        */
+#define RS_ECANCELED 6  /* serving: the request was cancelled while queued */
+#define RS_ETIMEDOUT 7  /* serving: the request outwaited the queue timeout */
 
 typedef void* rs_stream;
 
@@ -261,6 +263,18 @@ rs_status rs_server_result(rs_server* s, int64_t query_id, int32_t* h_docs, int6
                               int64_t* h_count);
 /* Block until every submitted query completed. */
 rs_status rs_server_drain(rs_server* s);
+/* Failure handling of the loop (no counterpart in the reference, which has
+ * no serving loop).  Cancel a request that is still queued: it completes at
+ * once with status RS_ECANCELED and count -1 and is never launched;
+ * RS_EINVAL if the id is not queued (already admitted, completed or
+ * unknown) -- an admitted request runs to completion. */
+rs_status rs_server_cancel(rs_server* s, int64_t query_id);
+/* Queue timeout: a request that has waited longer than `ms` when the server
+ * comes to admit it completes with RS_ETIMEDOUT and count -1, unlaunched.
+ * 0 (the default) disables the timeout. */
+rs_status rs_server_set_queue_timeout(rs_server* s, double ms);
+/* Pause (1) or resume (0) admission; batches already launched complete. */
+rs_status rs_server_pause(rs_server* s, int32_t paused);
 /* Launches (IR and lookup batches), completed requests and peak admitted
  * bytes so far. */
 rs_status rs_server_stats(const rs_server* s, int64_t* h_launches, int64_t* h_queries,

@@ -206,6 +206,8 @@ const char* rs_status_string(rs_status s) {
     case RS_EOVERFLOW: return "hash table overflow (probe bound reached)";
     case RS_EDEVICE: return "device error";
     case RS_EBUSY: return "busy";
+    case RS_ECANCELED: return "cancelled";
+    case RS_ETIMEDOUT: return "timed out in the queue";
     default: return "unknown status";
   }
 }
@@ -237,7 +239,8 @@ rs_status rs_calculate_lengths(const int32_t* d_offsets, int64_t num_sets, float
   RS_HIP(hipStreamSynchronize(st));
   if (h_max_length) *h_max_length = h[0];
   if (h[1]) {
-    set_error("rs_calculate_lengths: a set's length overflows int32 (degree too large)");
+    set_error("rs_calculate_lengths: a set's degree is negative (offsets decrease) or its length "
+              "overflows int32 (degree too large)");
     return RS_EINVAL;
   }
   return RS_OK;

@@ -267,6 +267,8 @@ struct rs_server {
   std::unordered_map<int64_t, std::vector<int32_t>> results;
   int64_t submitted = 0, completed = 0, launches = 0, admitted_bytes = 0, peak_bytes = 0;
   bool stop = false;
+  bool paused = false;           // admission held (rs_server_pause)
+  double queue_timeout_ms = 0.0; // 0 = none (rs_server_set_queue_timeout)
   rs_status worker_error = RS_OK;
   std::thread worker;
   // similarity lookups (second request kind), served by their own thread
@@ -367,6 +369,21 @@ static rs_status slot_launch(rs_server* S, IrSlot& s) {
   return RS_OK;
 }
 
+// With the server's lock held: complete, with RS_ETIMEDOUT, the queued
+// requests (oldest first) that waited longer than the queue timeout.
+static void expire_locked(rs_server* S, std::deque<IrPending>& q, int32_t kind) {
+  if (S->queue_timeout_ms <= 0.0) return;
+  const double t = now_ms();
+  bool any = false;
+  while (!q.empty() && t - q.front().submit_ms > S->queue_timeout_ms) {
+    S->done.push_back(rs_completion{q.front().id, -1, RS_ETIMEDOUT, q.front().submit_ms, t, kind, 0});
+    S->completed++;
+    q.pop_front();
+    any = true;
+  }
+  if (any) S->cv.notify_all();
+}
+
 static void server_loop(rs_server* S) {
   (void)hipSetDevice(S->device);
   while (true) {
@@ -406,9 +423,10 @@ static void server_loop(rs_server* S) {
     {
       std::unique_lock<std::mutex> lk(S->mu);
       if (S->stop) break;
+      if (!S->paused) expire_locked(S, S->pending, RS_REQ_IR);
       for (auto& sp : S->slots) {
         IrSlot& s = *sp;
-        if (s.state != 0 || S->pending.empty()) continue;
+        if (S->paused || s.state != 0 || S->pending.empty()) continue;
         int64_t bytes = 0;
         while (!S->pending.empty() && (int32_t)s.batch.size() < S->max_batch) {
           const IrPending& p = S->pending.front();
@@ -447,7 +465,7 @@ static void server_loop(rs_server* S) {
           S->cv.wait_for(lk, std::chrono::microseconds(20));
         else
           S->cv.wait_for(lk, std::chrono::milliseconds(2),
-                         [S] { return S->stop || !S->pending.empty(); });
+                         [S] { return S->stop || (!S->paused && !S->pending.empty()); });
       }
     }
   }
@@ -522,8 +540,9 @@ static void sss_loop(rs_server* S) {
   (void)hipSetDevice(S->device);
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
-    S->cv.wait(lk, [S] { return S->stop || !S->sss_pending.empty(); });
+    S->cv.wait(lk, [S] { return S->stop || (!S->paused && !S->sss_pending.empty()); });
     if (S->stop) break;
+    expire_locked(S, S->sss_pending, RS_REQ_SSS);
     // admit FIFO while the batch fits the shared memory budget
     std::vector<IrPending> batch;
     int64_t bytes = 0;
@@ -708,6 +727,42 @@ rs_status rs_server_result(rs_server* S, int64_t query_id, int32_t* h_docs, int6
   const int64_t n = *h_count < cap ? *h_count : cap;
   if (n > 0) memcpy(h_docs, it->second.data(), sizeof(int32_t) * (size_t)n);
   if (n == *h_count) S->results.erase(it);  // kept until fully copied out
+  return RS_OK;
+}
+
+rs_status rs_server_cancel(rs_server* S, int64_t query_id) {
+  RS_REQUIRE(S, "rs_server_cancel: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (auto* q : {&S->pending, &S->sss_pending}) {
+    for (auto it = q->begin(); it != q->end(); ++it) {
+      if (it->id != query_id) continue;
+      const int32_t kind = q == &S->pending ? RS_REQ_IR : RS_REQ_SSS;
+      S->done.push_back(rs_completion{it->id, -1, RS_ECANCELED, it->submit_ms, now_ms(), kind, 0});
+      S->completed++;
+      q->erase(it);
+      S->cv.notify_all();
+      return RS_OK;
+    }
+  }
+  set_error("rs_server_cancel: query %lld is not queued (admitted, completed or unknown)",
+            (long long)query_id);
+  return RS_EINVAL;
+}
+
+rs_status rs_server_set_queue_timeout(rs_server* S, double ms) {
+  RS_REQUIRE(S && ms >= 0.0, "rs_server_set_queue_timeout: bad arguments");
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->queue_timeout_ms = ms;
+  return RS_OK;
+}
+
+rs_status rs_server_pause(rs_server* S, int32_t paused) {
+  RS_REQUIRE(S, "rs_server_pause: null server");
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->paused = paused != 0;
+  }
+  S->cv.notify_all();
   return RS_OK;
 }
 

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libreset_amd.so")
 
 RS_OK, RS_EINVAL, RS_ENOMEM, RS_EOVERFLOW, RS_EDEVICE, RS_EBUSY = 0, 1, 2, 3, 4, 5
+RS_ECANCELED, RS_ETIMEDOUT = 6, 7
 REQ_IR, REQ_SSS = 0, 1
 NORMAL, HIERARCHICAL = 0, 1
 BUILD_NORMAL, BUILD_HIERARCHICAL, BUILD_BOTH = 1, 2, 3
@@ -91,6 +92,9 @@ SIGNATURES = {
     "rs_server_poll": (_i32, [_p, _p, _i32, _p]),
     "rs_server_result": (_i32, [_p, _i64, _p, _i64, _p]),
     "rs_server_drain": (_i32, [_p]),
+    "rs_server_cancel": (_i32, [_p, _i64]),
+    "rs_server_set_queue_timeout": (_i32, [_p, C.c_double]),
+    "rs_server_pause": (_i32, [_p, _i32]),
     "rs_server_stats": (_i32, [_p, _p, _p, _p]),
     "rs_server_free": (None, [_p]),
     "rs_pairs_default_opts": (None, [C.POINTER(rs_pairs_opts)]),

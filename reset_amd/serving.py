@@ -73,6 +73,20 @@ class Server:
     def drain(self):
         check(load().rs_server_drain(self._h))
 
+    def cancel(self, query_id):
+        """Cancel a still-queued request (it completes with status
+        RS_ECANCELED); raises ResetError if it is not queued."""
+        check(load().rs_server_cancel(self._h, int(query_id)))
+
+    def set_queue_timeout(self, ms):
+        """Requests that wait longer than `ms` before admission complete with
+        RS_ETIMEDOUT, unlaunched (0 disables)."""
+        check(load().rs_server_set_queue_timeout(self._h, float(ms)))
+
+    def pause(self, paused=True):
+        """Hold (or resume) admission; launched batches still complete."""
+        check(load().rs_server_pause(self._h, 1 if paused else 0))
+
     def stats(self):
         a, b, c = C.c_int64(0), C.c_int64(0), C.c_int64(0)
         check(load().rs_server_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))

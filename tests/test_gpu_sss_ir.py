@@ -263,3 +263,66 @@ def test_server_mixes_ir_queries_and_similarity_lookups(api, oracle_lib):
         bad.submit_sss(1, [1, 2, 3])  # no index attached
     sidx.close()
     t.close()
+
+
+def test_server_cancel_timeout_and_pause(api, oracle_lib):
+    """Failure handling forced from the host.  With admission paused, queued
+    requests can be cancelled (status RS_ECANCELED, never launched) or left
+    to outwait a queue timeout (RS_ETIMEDOUT); the others complete exactly as
+    served alone.  Cancelling an unknown or completed id is an error, and
+    both request kinds are covered."""
+    import time
+
+    from reset_amd import serving
+    from reset_amd.lib import RS_ECANCELED, RS_ETIMEDOUT, ResetError
+
+    rng = np.random.default_rng(41)
+    idx_off, docs, q, qoff = _ir_case(rng, nterms=200, ndocs=100000, maxlen=8000, nq=40, kmax=5)
+    queries = [q[qoff[i]:qoff[i + 1]].tolist() for i in range(40)]
+    m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
+    want = oracle_lib.ir_result_lists(m_res, m_cnt, m_roff)
+    soff, scols = G.zipf_sets(rng, 500, 300, 3, 30)
+    t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 1)
+    sidx = api.SSSIndex(_cuda(soff), _cuda(scols), 4, 5)
+    srv = serving.Server(t, budget_bytes=1 << 30, max_batch=8, max_inflight=2, sss_index=sidx)
+    # cancellation while paused
+    srv.pause()
+    for i in range(40):
+        srv.submit(i, queries[i])
+    srv.submit_sss(1000, scols[soff[5]:soff[6]].tolist())
+    for i in (3, 7, 1000):
+        srv.cancel(i)
+    with pytest.raises(ResetError):
+        srv.cancel(5000)  # unknown
+    srv.pause(False)
+    srv.drain()
+    comps = {c.query_id: c for c in srv.poll_all()}
+    assert len(comps) == 41
+    assert {i for i, c in comps.items() if c.status == RS_ECANCELED} == {3, 7, 1000}
+    assert comps[1000].kind == serving.REQ_SSS and comps[3].count == -1
+    for i in range(40):
+        if i not in (3, 7):
+            assert comps[i].status == 0 and srv.result(i) == want[i]
+    with pytest.raises(ResetError):
+        srv.cancel(0)  # completed
+    # queue timeout: requests held past 1 ms time out unlaunched
+    launches = srv.stats()["launches"]
+    srv.pause()
+    for i in range(100, 110):
+        srv.submit(i, queries[i - 100])
+    srv.submit_sss(2000, scols[soff[9]:soff[10]].tolist())
+    srv.set_queue_timeout(1.0)
+    time.sleep(0.05)
+    srv.pause(False)
+    srv.drain()
+    comps = {c.query_id: c for c in srv.poll_all()}
+    assert set(comps) == set(range(100, 110)) | {2000}
+    assert all(c.status == RS_ETIMEDOUT and c.count == -1 for c in comps.values())
+    assert srv.stats()["launches"] == launches
+    # timeout off again: requests run
+    srv.set_queue_timeout(0)
+    srv.submit(300, queries[0])
+    srv.drain()
+    (c,) = srv.poll_all()
+    assert c.status == 0 and srv.result(300) == want[0]
+    srv.close()

@@ -85,6 +85,21 @@ def test_table_overflow_is_reported(api):
     assert ei.value.status == 3
 
 
+def test_corrupted_inputs_are_rejected(api):
+    """Decreasing offsets and an element equal to the EMPTY marker -1 are
+    input errors (RS_EINVAL), found on the device and reported, not run."""
+    from reset_amd.lib import ResetError
+
+    with pytest.raises(ResetError) as ei:
+        api.HashTables(_cuda(np.array([0, 5, 3, 8], np.int32)), _cuda(np.arange(8, dtype=np.int32)),
+                       0.2, 5, 3)
+    assert ei.value.status == 1 and "negative" in str(ei.value)
+    with pytest.raises(ResetError) as ei:
+        api.HashTables(_cuda(np.array([0, 3, 6], np.int32)), _cuda(np.array([1, -1, 7, 2, 3, 4], np.int32)),
+                       0.2, 5, 3)
+    assert ei.value.status == 1 and "EMPTY" in str(ei.value)
+
+
 def test_probe_order_matches_oracle_on_same_table(api, oracle_lib):
     rng = np.random.default_rng(9)
     off, cols = G.random_sets(rng, 2000, 300, 1 << 20)
