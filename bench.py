@@ -657,12 +657,56 @@ def bench_compare(args):
     }
 
 
+def bench_pairs_shapes(args):
+    """Launch shapes of the batched pair kernel (waves per CU, LDS table slots
+    per wave, per-bin load factor and chunk) on the headline 1M pairs: kernel
+    ms (best of --steps launches, HIP events) per shape, all bit-checked.  A
+    comparison line, not the headline metric."""
+    import torch
+
+    from reset_amd import api, inputs
+
+    batch = inputs.make_pair_batch(args.pairs, seed=SEED)
+    _sync_all()
+    P = args.pairs
+    n = int(batch["a_off"][-1]) + int(batch["b_off"][-1])
+    shapes = [(16, 2048, 0.125, 255), (16, 2048, 0.25, 511), (8, 4096, 0.125, 511), (20, 1536, 0.25, 255),
+              (24, 1024, 0.25, 255)]
+    rows = []
+    for wpc, slots, alpha, chunk in shapes:
+        opts = api.default_pairs_opts()
+        opts.waves_per_cu, opts.lds_slots_per_wave = wpc, slots
+        for i in range(opts.num_bins):
+            opts.bins[i].load_factor, opts.bins[i].chunk, opts.bins[i].unroll = alpha, chunk, 2
+        plan = api.PairsPlan(P, opts)
+        sizes = torch.empty(P, dtype=torch.int32, device="cuda")
+        plan.run_async(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes)
+        plan.set_timing(True)
+        ms = []
+        for _ in range(max(1, args.steps)):
+            plan.run_async(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes)
+            ms.append(plan.kernel_ms())
+        plan.status()
+        rows.append({"waves_per_cu": wpc, "lds_slots_per_wave": slots, "load_factor": alpha, "chunk": chunk,
+                     "unroll": 2, "kernel_ms": min(ms), "alg_TBps": 4 * n / min(ms) / 1e9,
+                     "bit_exact": bool(torch.equal(sizes, batch["k"]))})
+        plan.close()
+    best = min(rows, key=lambda r: r["kernel_ms"])
+    return {"metric": "batched pair kernel ms per launch shape (1 GPU)", "value": best["kernel_ms"], "unit": "ms",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": best["kernel_ms"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic (the headline pairs)",
+            "config": {"workload": f"batched pairwise intersection, {P} pairs, launch-shape comparison",
+                       "pairs": P, "parallelism": "single GPU"},
+            "shapes": rows}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pairs", choices=["pairs", "tc", "sss", "ir", "compare"])
+    ap.add_argument("--workload", default="pairs", choices=["pairs", "tc", "sss", "ir", "compare", "pairs_shapes"])
     ap.add_argument("--sets", type=int, default=1_000_000)
     ap.add_argument("--queries", type=int, default=20_000)
     ap.add_argument("--qps", type=float, default=2000.0)
@@ -711,7 +755,7 @@ def main(argv=None):
     if args.scale is None:
         args.scale = 20 if args.workload == "compare" else 24
     out = {"pairs": bench_pairs, "tc": bench_tc, "sss": bench_sss, "ir": bench_ir,
-           "compare": bench_compare}[args.workload](args)
+           "compare": bench_compare, "pairs_shapes": bench_pairs_shapes}[args.workload](args)
     print(json.dumps(out), flush=True)
     return 0
 
