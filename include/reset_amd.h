@@ -303,7 +303,16 @@ typedef struct rs_pairs_opts {
   int32_t lds_slots_per_wave;       /* int32 LDS slots of a wave's table;   */
                                     /* (slots + 512 staging) x 4 B x waves  */
                                     /* per CU must fit the 160 KiB LDS      */
+  int32_t schedule;                 /* 0 static, 1..64 work stealing        */
 } rs_pairs_opts;
+/* How waves of the batched launch take pairs from the size-sorted order:
+ * schedule 0 (RS_PAIRS_STATIC), wave w takes positions w, w + W, ... (no
+ * atomics); schedule k in 1..64, each wave takes runs of k consecutive
+ * positions from a shared counter, the reference's work stealing
+ * (TC.cu:141-142,176-182, CHUNK = k).  The default is RS_PAIRS_STEAL, runs
+ * of 8. */
+#define RS_PAIRS_STATIC 0
+#define RS_PAIRS_STEAL 8
 /* Defaults tuned for gfx950 (also the autotuner's starting point). */
 void rs_pairs_default_opts(rs_pairs_opts* o);
 
@@ -318,7 +327,8 @@ rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* 
                                          int32_t bucket_size, int32_t* d_workspace,
                                          int32_t* d_sizes, rs_stream stream);
 
-/* The batched form: one persistent, size-binned, work-stealing launch.  A
+/* The batched form: one persistent, size-binned launch whose waves take pairs
+ * statically or by work stealing (opts.schedule).  A
  * plan bins the pairs (largest first) once per batch; the launch builds each
  * pair's smaller set into an LDS table and streams the larger set through
  * it; a smaller set above the bin's chunk is built chunk by chunk in value

@@ -5,7 +5,8 @@
 //
 // Batched form: one persistent launch over all pairs.  Each wave owns a
 // private LDS table region and takes the pairs of a size-binned plan (largest
-// bins first) round-robin.  Per pair it builds the SMALLER set into an LDS
+// bins first) by work stealing, in runs of consecutive pairs (or round-robin,
+// on request).  Per pair it builds the SMALLER set into an LDS
 // table (bucket-major, the reference's bucket rule) and streams the LARGER
 // set through it once, so HBM sees each input element once (the algorithmic
 // minimum 4(|A|+|B|) bytes).  A smaller set above the bin's chunk is built
@@ -31,6 +32,7 @@ struct PairBinDev {
 struct PairsConst {
   PairBinDev bins[RS_PAIRS_MAX_BINS];
   int32_t num_bins;
+  int32_t steal;  // > 0: waves take this many positions a time from the shared counter
 };
 
 __device__ __forceinline__ int pair_bin(int64_t total, int num_bins) {
@@ -86,12 +88,14 @@ struct PairSrcIdx {
 
 // ---------------------------------------------------------------- plan
 // Layout of the plan's control block (int32): [0..kMaxBins) histogram,
-// [kMaxBins..2kMaxBins) cursors, [2kMaxBins] error bits.  Reset by a kernel
-// each launch (no memset inside graphs).
+// [kMaxBins..2kMaxBins) cursors, [2kMaxBins] the work-stealing counter,
+// [2kMaxBins+1] error bits.  Reset by a kernel each launch (no memset inside
+// graphs).
 constexpr int kCtlHist = 0;
 constexpr int kCtlCursor = RS_PAIRS_MAX_BINS;
-constexpr int kCtlStatus = 2 * RS_PAIRS_MAX_BINS;
-constexpr int kCtlWords = 2 * RS_PAIRS_MAX_BINS + 1;
+constexpr int kCtlNext = 2 * RS_PAIRS_MAX_BINS;
+constexpr int kCtlStatus = 2 * RS_PAIRS_MAX_BINS + 1;
+constexpr int kCtlWords = 2 * RS_PAIRS_MAX_BINS + 2;
 
 __global__ void plan_reset(int32_t* ctl) {
   if (threadIdx.x < kCtlWords - 1) ctl[threadIdx.x] = 0;  // keeps the error word
@@ -493,10 +497,13 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
   return hits;
 }
 
-// Persistent kernel: wave w takes the pairs at positions w, w + W, w + 2W, ...
-// of the size-sorted order (largest first), so every wave gets a near-equal
-// mix and no global atomic is needed; the next pair's offsets are fetched
-// before the current pair is processed.
+// Persistent kernel over the size-sorted order (largest first).  Static
+// schedule: wave w takes the positions w, w + W, w + 2W, ..., so every wave
+// gets a near-equal mix and no global atomic is needed.  Work stealing
+// (TC.cu:141-142,176-182 takes chunks of edges the same way): lane 0 of a
+// wave takes the next run of K consecutive positions from the control
+// block's counter, one run ahead of the run in use.  Either way the next
+// pair's offsets are fetched before the current pair is processed.
 template <int B, int U, int CPL, class Src>
 __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     const Src src, int64_t P, const int32_t* __restrict__ order, int32_t* ctl,
@@ -513,7 +520,34 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
   __builtin_amdgcn_wave_barrier();
   const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
-  int64_t idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
+  const int32_t K = cfg.steal;
+  // the counter counts runs: at most P / K + 2W takes, below 2^32
+  auto take = [&]() -> int64_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(reinterpret_cast<uint32_t*>(ctl + kCtlNext), 1u);
+    return (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)v) * K;
+  };
+  // stealing: the run in use starts at q0 (r positions of it handed out),
+  // the next run at q1
+  int64_t q0 = 0, q1 = 0;
+  int32_t r = 0;
+  auto following = [&](int64_t cur) -> int64_t {
+    if (K == 0) return cur + W;
+    if (++r < K) return q0 + r;
+    r = 0;
+    q0 = q1;
+    q1 = take();
+    return q0;
+  };
+  int64_t idx;
+  if (K > 0) {
+    q0 = take();
+    q1 = take();
+    idx = q0;
+  } else {
+    idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
+  }
+  int64_t nidx = following(idx);
   bool bad = false;
   int32_t i = 0;
   const int32_t* A = nullptr;
@@ -523,9 +557,10 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     i = order[idx];
     src.get(i, A, na, Bp, nb);
   }
-  for (; idx < P; idx += W) {
-    // prefetch the next pair's metadata
-    const int64_t nidx = idx + W;
+  while (idx < P) {
+    // the position after next (its atomic completes during this pair), and
+    // the next pair's metadata
+    const int64_t nnidx = following(nidx);
     int32_t ni = 0;
     const int32_t* nA = nullptr;
     const int32_t* nB = nullptr;
@@ -566,6 +601,8 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     Bp = nB;
     na = nna;
     nb = nnb;
+    idx = nidx;
+    nidx = nnidx;
   }
   if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
 }
@@ -645,7 +682,11 @@ static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   RS_REQUIRE((int64_t)(o->lds_slots_per_wave + kRing) * 4 * o->waves_per_cu <= 160 * 1024,
              "pairs: %d waves x (%d table + %d staging) slots exceed the 160 KiB LDS of a CU",
              o->waves_per_cu, o->lds_slots_per_wave, kRing);
+  RS_REQUIRE(o->schedule >= 0 && o->schedule <= 64,
+             "pairs: schedule must be 0 (static) or 1..64 pairs per take (work stealing), not %d",
+             o->schedule);
   cfg->num_bins = o->num_bins;
+  cfg->steal = o->schedule;
   for (int i = 0; i < o->num_bins; ++i) {
     const rs_pairs_bin& bn = o->bins[i];
     RS_REQUIRE(bn.load_factor > 0.f && (bn.bucket_size == 1 || bn.bucket_size == 2 ||
@@ -754,6 +795,11 @@ void rs_pairs_default_opts(rs_pairs_opts* o) {
   o->num_bins = RS_PAIRS_MAX_BINS;
   o->waves_per_cu = 16;
   o->lds_slots_per_wave = 2048;
+  // work stealing in runs of 8: a pair's cost depends on its size ratio, not
+  // only on its size, so a static deal of ~250 pairs per wave leaves the
+  // slowest wave ~20% behind the mean; runs of 8 keep the shared counter
+  // uncontended (one take per pair stalls on it)
+  o->schedule = RS_PAIRS_STEAL;
   for (int i = 0; i < RS_PAIRS_MAX_BINS; ++i) {
     // alpha 1/8, b 4: L = nextpow2 > 2*chunk, so a 255-element chunk fills
     // the 2048-slot region at <= 1/2 key per 4-slot bucket, where a home

@@ -17,6 +17,7 @@ REQ_IR, REQ_SSS = 0, 1
 NORMAL, HIERARCHICAL = 0, 1
 BUILD_NORMAL, BUILD_HIERARCHICAL, BUILD_BOTH = 1, 2, 3
 PAIRS_MAX_BINS = 20
+RS_PAIRS_STATIC, RS_PAIRS_STEAL = 0, 8  # rs_pairs_opts.schedule: static, or pairs per work-stealing take
 
 
 class ResetError(RuntimeError):
@@ -43,7 +44,7 @@ class rs_pairs_bin(C.Structure):
 
 class rs_pairs_opts(C.Structure):
     _fields_ = [("num_bins", C.c_int32), ("bins", rs_pairs_bin * PAIRS_MAX_BINS),
-                ("waves_per_cu", C.c_int32), ("lds_slots_per_wave", C.c_int32)]
+                ("waves_per_cu", C.c_int32), ("lds_slots_per_wave", C.c_int32), ("schedule", C.c_int32)]
 
 
 class rs_completion(C.Structure):
