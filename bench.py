@@ -40,6 +40,34 @@ def _cpu_model():
     return None
 
 
+_FLUSH = []
+FLUSH_BYTES = 1 << 30
+
+
+def _flush_caches():
+    """Write 1 GiB on the current stream, so that nothing a kernel reads is
+    left in the eight 4 MiB L2s or the 256 MiB MALL (the reference flushes
+    before timed reps too: baseline/common/utils.cuh:25-41, 60-80)."""
+    import torch
+
+    if not _FLUSH:
+        _FLUSH.append(torch.empty(FLUSH_BYTES // 4, dtype=torch.int32, device="cuda"))
+    _FLUSH[0].fill_(0x5A5A5A5A)
+
+
+def _cold(run_timed, reps):
+    """Median over reps of run_timed() (which enqueues one timed run and
+    returns its kernel ms) with the caches flushed before each rep."""
+    import numpy as np
+
+    ms = []
+    for _ in range(reps):
+        _flush_caches()
+        ms.append(run_timed())
+    return {"kernel_ms_median": float(np.median(ms)), "kernel_ms_min": float(np.min(ms)), "reps": reps,
+            "flush_bytes": FLUSH_BYTES}
+
+
 def _traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_traffic.json")
@@ -103,6 +131,11 @@ def bench_pairs(args):
     for _ in range(args.steps):
         plan.run_async(a_off, a, b_off, b, sizes)
         kms.append(plan.kernel_ms())
+    def _one():
+        plan.run_async(a_off, a, b_off, b, sizes)
+        return plan.kernel_ms()
+
+    cold = _cold(_one, max(10, args.steps))
     plan.set_timing(False)
     parity = parity and bool(torch.equal(sizes, k))
     kernel_ms = float(np.mean(kms))
@@ -140,6 +173,7 @@ def bench_pairs(args):
             "kernel_ms": kernel_ms,
             "alg_bytes_per_launch": alg_bytes,
         },
+        "cold_cache": cold,
         "gen_s": gen_s,
         "autotune": tune,
     }
@@ -224,7 +258,17 @@ def bench_tc(args):
         _sync_all()
         t.status()
         res[name] = (e0.elapsed_time(e1) / steps, int(out_t.item()))
-    counts = {v[1] for v in res.values()}
+    def _one():
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        t.tc_count_async(out_t, 1, keys, False)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    cold = _cold(_one, 10)
+    counts = {v[1] for v in res.values()} | {int(out_t.item())}
     ms = res["tiled_hierarchical"][0]
     tri = res["tiled_hierarchical"][1]
     alg_bytes = 8 * probes + 24 * edges
@@ -250,6 +294,7 @@ def bench_tc(args):
         "parity": {"layouts_agree": len(counts) == 1},
         "pairs_per_s": edges / (ms / 1000.0),
         "variants_ms": {k: v[0] for k, v in res.items()},
+        "cold_cache": cold,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _tc_traffic(args.scale, alg_bytes),
                      "kernel": "tc_staged_kernel + tc_tiled_kernel", "kernel_ms": ms,
