@@ -64,6 +64,8 @@ def _declare(L):
     L.orc_tc_hash.argtypes = [_p, _p, _i64, _p, _p, _i32, _i32, _i32, _p, C.c_int, _p]
     L.orc_tc_merge.restype = C.c_uint64
     L.orc_tc_merge.argtypes = [_p, _p, _i64, C.c_int]
+    L.orc_tc_hash_rows.restype = C.c_int
+    L.orc_tc_hash_rows.argtypes = [_p, _p, _i64, _i64, _i64, _p, _p, _i32, _i32, _i32, _p, C.c_int, _p]
     L.orc_tc_merge_rows.restype = C.c_uint64
     L.orc_tc_merge_rows.argtypes = [_p, _p, _i64, _i64, C.c_int]
     L.orc_pairs_hash.restype = C.c_int
@@ -199,6 +201,18 @@ def tc_hash(off, cols, tables: Tables, probe_cols=None, nthreads=0) -> int:
                            C.byref(out))
     if rc:
         raise OracleError(f"tc rc={rc}")
+    return int(out.value)
+
+
+def tc_hash_rows(off, cols, tables: Tables, row_begin, row_end, nthreads=0) -> int:
+    """tc_hash over the edges whose source row lies in [row_begin, row_end)."""
+    off, cols = _c(off, np.int32), _c(cols, np.int32)
+    out = C.c_uint64(0)
+    rc = lib().orc_tc_hash_rows(_ptr(off), _ptr(cols), len(off) - 1, int(row_begin), int(row_end), _ptr(tables.L),
+                                _ptr(tables.toff), tables.bucket_size, tables.max_len, tables.layout, _ptr(tables.T),
+                                nthreads, C.byref(out))
+    if rc:
+        raise OracleError(f"tc rows rc={rc}")
     return int(out.value)
 
 

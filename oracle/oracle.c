@@ -256,22 +256,18 @@ static int64_t probe_row(const int32_t* off, const int32_t* keys, int32_t u,
   return hits;
 }
 
-/* TC.cu:128-190: edge e = (vertexs[e], cols[e]); swap so the smaller out-list
- * probes the larger one's table (TC.cu:146-155); chunk-4 dynamic schedule
- * (TC.cu:141-182). */
-int orc_tc_hash(const int32_t* off, const int32_t* cols, int64_t n,
-                const int32_t* L, const int64_t* toff, int32_t bucket_size,
-                int32_t max_len, int32_t layout, const int32_t* T,
-                int nthreads, uint64_t* out) {
-  const int64_t E = off[n];
-  const int64_t BN = toff[n];
-  int32_t* rows = expand_rows(off, n);
+/* TC.cu:128-190 over the edges [e_begin, e_end): edge e = (rows[e - e_begin],
+ * cols[e]); swap so the smaller out-list probes the larger one's table
+ * (TC.cu:146-155); chunk-4 dynamic schedule (TC.cu:141-182). */
+static int tc_hash_edges(const int32_t* off, const int32_t* cols, const int32_t* rows,
+                         int64_t e_begin, int64_t e_end, int64_t BN, const int32_t* L,
+                         const int64_t* toff, int32_t bucket_size, int32_t max_len,
+                         int32_t layout, const int32_t* T, uint64_t* out) {
   uint64_t total = 0;
   int failed = 0;
-  set_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : total) reduction(| : failed)
-  for (int64_t e = 0; e < E; e++) {
-    int32_t u = rows[e], v = cols[e];
+  for (int64_t e = e_begin; e < e_end; e++) {
+    int32_t u = rows[e - e_begin], v = cols[e];
     if (off[u + 1] - off[u] > off[v + 1] - off[v]) {
       const int32_t t = u;
       u = v;
@@ -284,9 +280,38 @@ int orc_tc_hash(const int32_t* off, const int32_t* cols, int64_t n,
     else
       total += (uint64_t)h;
   }
-  free(rows);
   *out = total;
   return failed ? ORC_EOVERFLOW : ORC_OK;
+}
+
+int orc_tc_hash(const int32_t* off, const int32_t* cols, int64_t n,
+                const int32_t* L, const int64_t* toff, int32_t bucket_size,
+                int32_t max_len, int32_t layout, const int32_t* T,
+                int nthreads, uint64_t* out) {
+  int32_t* rows = expand_rows(off, n);
+  set_threads(nthreads);
+  const int rc = tc_hash_edges(off, cols, rows, 0, off[n], toff[n], L, toff, bucket_size,
+                               max_len, layout, T, out);
+  free(rows);
+  return rc;
+}
+
+/* orc_tc_hash restricted to the edges of source rows [row_begin, row_end)
+ * (a bounded sample of a large graph; every table is still needed). */
+int orc_tc_hash_rows(const int32_t* off, const int32_t* cols, int64_t n, int64_t row_begin,
+                     int64_t row_end, const int32_t* L, const int64_t* toff,
+                     int32_t bucket_size, int32_t max_len, int32_t layout, const int32_t* T,
+                     int nthreads, uint64_t* out) {
+  if (row_begin < 0 || row_end > n || row_begin > row_end) return ORC_EINVAL;
+  const int64_t e0 = off[row_begin], e1 = off[row_end];
+  int32_t* rows = (int32_t*)malloc(sizeof(int32_t) * (size_t)(e1 > e0 ? e1 - e0 : 1));
+  for (int64_t t = row_begin; t < row_end; t++)
+    for (int32_t e = off[t]; e < off[t + 1]; e++) rows[e - e0] = (int32_t)t;
+  set_threads(nthreads);
+  const int rc = tc_hash_edges(off, cols, rows, e0, e1, toff[n], L, toff, bucket_size,
+                               max_len, layout, T, out);
+  free(rows);
+  return rc;
 }
 
 static int64_t merge_count(const int32_t* a, int64_t na, const int32_t* b, int64_t nb) {

@@ -80,6 +80,11 @@ int orc_tc_hash(const int32_t* off, const int32_t* cols, int64_t n,
                 const int32_t* L, const int64_t* toff, int32_t bucket_size,
                 int32_t max_len, int32_t layout, const int32_t* T,
                 int nthreads, uint64_t* out);
+/* orc_tc_hash over the edges of source rows [row_begin, row_end) only. */
+int orc_tc_hash_rows(const int32_t* off, const int32_t* cols, int64_t n, int64_t row_begin,
+                     int64_t row_end, const int32_t* L, const int64_t* toff,
+                     int32_t bucket_size, int32_t max_len, int32_t layout, const int32_t* T,
+                     int nthreads, uint64_t* out);
 
 /* Independent textbook check: sum over edges of |N+(u) cap N+(v)| by sorted
  * merge (rows must be sorted ascending). */

@@ -331,3 +331,12 @@ def test_tc_row_ranges_add_up(oracle_lib):
         cuts = [0, 37, 120, 121, 250, n]
         parts = [oracle_lib.tc_merge_rows(off, cols, a, b) for a, b in zip(cuts, cuts[1:])]
         assert sum(parts) == total == G.brute_triangles(und, n)
+        # the hash count over the same row ranges: range by range equal to the
+        # merge count, in both layouts
+        for layout in (0, 1):
+            t = oracle_lib.build_tables(off, cols, 0.2, 5, layout)
+            hparts = [oracle_lib.tc_hash_rows(off, cols, t, a, b) for a, b in zip(cuts, cuts[1:])]
+            assert hparts == parts
+            assert oracle_lib.tc_hash_rows(off, cols, t, 0, n) == oracle_lib.tc_hash(off, cols, t) == total
+    with pytest.raises(oracle_lib.OracleError):
+        oracle_lib.tc_hash_rows(off, cols, t, 5, n + 1)
