@@ -742,18 +742,20 @@ def bench_pairs_shapes(args):
     _sync_all()
     P = args.pairs
     n = int(batch["a_off"][-1]) + int(batch["b_off"][-1])
-    # (waves/CU, table slots/wave, load factor, chunk, schedule: 0 static, k work stealing k pairs a take)
-    shapes = [(16, 2048, 0.125, 255, 0), (16, 2048, 0.125, 255, 1), (16, 2048, 0.125, 255, 4),
-              (16, 2048, 0.125, 255, 8), (16, 2048, 0.125, 255, 16), (16, 2048, 0.125, 255, 32),
-              (16, 2048, 0.125, 255, 64), (16, 2048, 0.25, 511, 0), (16, 2048, 0.25, 511, 16),
-              (8, 4096, 0.125, 511, 0), (8, 4096, 0.125, 511, 16), (20, 1536, 0.25, 255, 0),
-              (20, 1536, 0.25, 255, 16), (24, 1024, 0.25, 255, 0), (24, 1024, 0.25, 255, 16)]
+    # (waves/CU, table slots/wave, load factor, chunk, schedule: 0 static, k work stealing k pairs a take,
+    #  unroll)
+    shapes = [(16, 2048, 0.125, 255, 0, 2), (16, 2048, 0.125, 255, 1, 2), (16, 2048, 0.125, 255, 4, 2),
+              (16, 2048, 0.125, 255, 8, 2), (16, 2048, 0.125, 255, 16, 2), (16, 2048, 0.125, 255, 32, 2),
+              (16, 2048, 0.125, 255, 64, 2), (16, 2048, 0.125, 255, 8, 1), (16, 2048, 0.125, 255, 8, 4),
+              (16, 2048, 0.25, 511, 0, 2), (16, 2048, 0.25, 511, 16, 2), (8, 4096, 0.125, 511, 0, 2),
+              (8, 4096, 0.125, 511, 16, 2), (20, 1536, 0.25, 255, 0, 2), (20, 1536, 0.25, 255, 16, 2),
+              (24, 1024, 0.25, 255, 0, 2), (24, 1024, 0.25, 255, 16, 2)]
     rows = []
-    for wpc, slots, alpha, chunk, sched in shapes:
+    for wpc, slots, alpha, chunk, sched, unroll in shapes:
         opts = api.default_pairs_opts()
         opts.waves_per_cu, opts.lds_slots_per_wave, opts.schedule = wpc, slots, sched
         for i in range(opts.num_bins):
-            opts.bins[i].load_factor, opts.bins[i].chunk, opts.bins[i].unroll = alpha, chunk, 2
+            opts.bins[i].load_factor, opts.bins[i].chunk, opts.bins[i].unroll = alpha, chunk, unroll
         plan = api.PairsPlan(P, opts)
         sizes = torch.empty(P, dtype=torch.int32, device="cuda")
         plan.run_async(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes)
@@ -764,7 +766,7 @@ def bench_pairs_shapes(args):
             ms.append(plan.kernel_ms())
         plan.status()
         rows.append({"waves_per_cu": wpc, "lds_slots_per_wave": slots, "load_factor": alpha, "chunk": chunk,
-                     "unroll": 2, "schedule": f"steal {sched}" if sched else "static", "kernel_ms": min(ms), "alg_TBps": 4 * n / min(ms) / 1e9,
+                     "unroll": unroll, "schedule": f"steal {sched}" if sched else "static", "kernel_ms": min(ms), "alg_TBps": 4 * n / min(ms) / 1e9,
                      "bit_exact": bool(torch.equal(sizes, batch["k"]))})
         plan.close()
     best = min(rows, key=lambda r: r["kernel_ms"])
