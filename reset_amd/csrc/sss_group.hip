@@ -78,12 +78,25 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
     const int32_t* __restrict__ b_off, const int32_t* __restrict__ b_el,
     const int32_t* __restrict__ cu, const int32_t* __restrict__ cv,
     const int64_t* __restrict__ unit_start, int64_t nunits, int64_t ncand,
-    int32_t* __restrict__ sizes, int32_t* __restrict__ err) {
+    int32_t* __restrict__ sizes, int32_t* __restrict__ err,
+    unsigned long long* __restrict__ next_unit) {
   __shared__ __attribute__((aligned(16))) int32_t tab[kGroupSlots];
+  __shared__ int64_t s_next;
   const int lane = __lane_id();
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   bool bad = false;
-  for (int64_t k = blockIdx.x; k < nunits; k += gridDim.x) {
+  // Work stealing (TC.cu:176-182 takes its chunks the same way): a unit's
+  // cost varies with its build set and its candidates, so workgroups take
+  // units from a shared counter instead of a fixed deal.  Thread 0 takes the
+  // next unit when a unit starts and publishes it after the build barrier
+  // (every thread read the previous value before the first barrier), so the
+  // atomic returns during the build; all threads read it after the unit.
+  if (threadIdx.x == 0) s_next = (int64_t)atomicAdd(next_unit, 1ull);
+  __syncthreads();
+  int64_t k = s_next;
+  while (k < nunits) {
+    unsigned long long taken = 0;
+    if (threadIdx.x == 0) taken = atomicAdd(next_unit, 1ull);
     const int64_t c0 = unit_start[k];
     const int64_t c1 = k + 1 < nunits ? unit_start[k + 1] : ncand;
     const int32_t u = cu[c0];
@@ -121,6 +134,7 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
       bad |= step == nslots;
     }
     __syncthreads();
+    if (threadIdx.x == 0) s_next = (int64_t)taken;
     if (ncw > 0) {  // wave-uniform
       auto meta = [&](const int32_t (&m)[4], int32_t t) {
         const int32_t r = t >> 6;
@@ -222,9 +236,12 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
       }
     }
     __syncthreads();
+    k = s_next;
   }
   if (__any(bad) && lane == 0) atomicOr(err, kErrTableFull);
 }
+
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 // Which candidates the grouped path takes: build set no larger than
 // kGroupMaxBuild.  sel[i] = 1 for those.
@@ -266,9 +283,13 @@ rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t*
   RS_HIP(hipMemcpyAsync(&nunits, nsel.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
   RS_HIP(hipStreamSynchronize(st));
   const int grid = device_cu_count() * 4;  // 32 KiB of LDS each: 4 workgroups per CU
+  // the work-stealing counter: the second word of nsel (the first holds the
+  // select count read above)
+  auto* next_unit = reinterpret_cast<unsigned long long*>(nsel.as<size_t>() + 1);
+  zero_u64_kernel<<<1, 1, 0, st>>>(next_unit);
   group_verify_kernel<<<grid, kGroupBlock, 0, st>>>(a_off, a_el, b_off, b_el, cu, cv,
                                                     starts.as<int64_t>(), (int64_t)nunits, n, sizes,
-                                                    d_err);
+                                                    d_err, next_unit);
   RS_HIP(hipGetLastError());
   RS_HIP(hipStreamSynchronize(st));  // flags/starts are released on return
   return RS_OK;
