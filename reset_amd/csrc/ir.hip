@@ -23,6 +23,7 @@ namespace rs {
 
 constexpr int kIrBlock = 256;
 constexpr int64_t kIrSeg = 4096;
+constexpr int64_t kIrRun = 4;  // segments per work-stealing take
 
 // per query: number of segments of term0's list (0 when it has no term or an
 // empty term0 list); validates term ids.
@@ -57,13 +58,35 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
     const int64_t* __restrict__ toff, const int32_t* __restrict__ T, int64_t BN, int32_t b,
     uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff, int64_t Q,
     const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t total_segs,
-    int32_t* ws, int32_t* __restrict__ seg_count, int32_t* __restrict__ status) {
+    int32_t* ws, int32_t* __restrict__ seg_count, int32_t* __restrict__ status,
+    unsigned long long* __restrict__ next_run) {
   (void)toff32_unused;
   const int lane = __lane_id();
+  bool bad = false;
+  // With no more segments than waves (a serving batch), wave w takes segment
+  // w.  With more, work stealing (ir.cu:94-176 takes chunks of queries the
+  // same way): a segment's cost varies with its query's term count and
+  // survivors, so waves take runs of kIrRun consecutive segments from a
+  // shared counter, the next run one run ahead so that its atomic returns
+  // meanwhile.
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  bool bad = false;
-  for (int64_t g = wave; g < total_segs; g += nwaves) {
+  const bool steal = total_segs > nwaves;
+  auto take = [&]() -> int64_t {
+    unsigned long long v = 0;
+    if (lane == 0) v = atomicAdd(next_run, 1ull);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo) * kIrRun;
+  };
+  int64_t q0 = 0, q1 = 0, g = wave;
+  int32_t r = 0;
+  if (steal) {
+    q0 = take();
+    q1 = take();
+    g = q0;
+  }
+  while (g < total_segs) {
     // query owning segment g: largest qi with segoff[qi] <= g
     int64_t lo = 0, hi = Q;
     while (hi - lo > 1) {
@@ -107,9 +130,20 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
       src = dst;
     }
     if (lane == 0) seg_count[g] = cur;
+    if (!steal) break;
+    if (++r < kIrRun) {
+      g = q0 + r;
+    } else {
+      r = 0;
+      q0 = q1;
+      q1 = take();
+      g = q0;
+    }
   }
   if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
 }
+
+__global__ void ir_reset_run_kernel(unsigned long long* p) { *p = 0ull; }
 
 // One wave per query: concatenate the survivors of its segments into
 // result[roff[q] ..] and write count[q].
@@ -136,6 +170,7 @@ __global__ void __launch_bounds__(kIrBlock) ir_gather_kernel(
 // Workspace-carrying enqueue used by the C entry point and by the server.
 struct IrScratch {
   DevBuf nseg, segoff, seg_count, tmp;
+  DevBuf next_run;  // the segment kernel's work-stealing counter
   int64_t total_segs = 0;
 };
 
@@ -163,17 +198,20 @@ static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* t
   if (sc.seg_count.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
     RS_TRY(sc.seg_count.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
   if (total > 0) {
+    if (!sc.next_run.p) RS_TRY(sc.next_run.alloc(sizeof(unsigned long long)));
+    auto* next_run = sc.next_run.as<unsigned long long>();
+    ir_reset_run_kernel<<<1, 1, 0, st>>>(next_run);
     const int grid = grid_for(total * kWave, kIrBlock, 8);
     if (layout == RS_HIERARCHICAL)
       ir_segment_kernel<RS_HIERARCHICAL><<<grid, kIrBlock, 0, st>>>(
           t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
           t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
-          sc.seg_count.as<int32_t>(), t->d_status());
+          sc.seg_count.as<int32_t>(), t->d_status(), next_run);
     else
       ir_segment_kernel<RS_NORMAL><<<grid, kIrBlock, 0, st>>>(
           t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
           t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
-          sc.seg_count.as<int32_t>(), t->d_status());
+          sc.seg_count.as<int32_t>(), t->d_status(), next_run);
   }
   ir_gather_kernel<<<grid_for(Q * kWave, kIrBlock, 8), kIrBlock, 0, st>>>(
       roff, sc.segoff.as<int64_t>(), Q, ws, sc.seg_count.as<int32_t>(), result, count);
