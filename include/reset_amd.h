@@ -133,7 +133,10 @@ rs_status rs_tc_plan_work(const rs_tc_plan* p, int64_t* h_probes, int64_t* h_edg
  * kernel reads every table slot from global memory, TC.cu:11-31). */
 rs_status rs_tc_plan_staged(const rs_tc_plan* p, int64_t* h_staged_probes, int64_t* h_units,
                             int64_t* h_tables_max_len);
-/* Tiled wave64 count into *d_count (uint64, overwritten). */
+/* Tiled wave64 count into *d_count (uint64, overwritten).  The plan holds
+ * the work-stealing counters of its kernels (and of the plain form), reset
+ * by a kernel at the start of every count: counts with one plan must not
+ * overlap on the device (use a plan per stream). */
 rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
                             const int32_t* d_probe_keys, uint64_t* d_count, rs_stream stream);
 /* The plain form: one wave per edge, CHUNK-edge work stealing, like the

@@ -52,6 +52,7 @@ struct rs_tc_plan {
   int64_t num_units = 0, num_tiles = 0;
   rs::DevBuf u_g, u_wb, u_we, u_tile, tile_edge;
   rs::DevBuf xcd_units;       // int64[kTcStagedClasses][kXcds+1]: unit split per XCD
+  rs::DevBuf unit_next;       // uint64[kTcStagedClasses][kXcds]: staged work-stealing counters
   int32_t class_lmax[kTcStagedClasses] = {0, 0, 0};    // largest staged L per class (0 = none)
   int64_t class_units[kTcStagedClasses + 1] = {0, 0, 0, 0};
 };
@@ -90,6 +91,9 @@ inline int32_t stage_lmax(int32_t b) {
 }
 
 __global__ void zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void zero_u64_n(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
 __global__ void set_i32(int32_t* p, int32_t v) { *p = v; }
 
 // Schedule endpoints of CSR edge e = (rows[e], cols[e]): s has the smaller
@@ -493,8 +497,10 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
     const int64_t* __restrict__ u_wb, const int64_t* __restrict__ u_we,
     const int64_t* __restrict__ u_tile, const int32_t* __restrict__ tile_edge,
     const int64_t* __restrict__ xcd_units, int32_t table_words,
-    unsigned long long* __restrict__ out, int32_t* __restrict__ status) {
+    unsigned long long* __restrict__ out, int32_t* __restrict__ status,
+    unsigned long long* __restrict__ unit_next) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_tc[];
+  __shared__ int64_t s_next;
   int32_t* const tab = lds_tc;
   const int lane = __lane_id();
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -503,11 +509,22 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
   for (int u = 0; u < U; ++u) marks[u * kWave + lane] = -1;
   int32_t tag = 0;
   const int xcd = blockIdx.x % kXcds;
-  const int64_t per = gridDim.x / kXcds;
   const int64_t ub = xcd_units[xcd], ue = xcd_units[xcd + 1];
+  // Work stealing within the XCD's unit range (TC.cu:176-182 takes edge
+  // chunks the same way), one counter per XCD so that the units an XCD's L2
+  // serves stay together: a unit's cost varies with its table and its
+  // probes' lists.  Thread 0 takes the next unit when a unit starts and
+  // publishes it after the table barrier (every thread read the previous
+  // value before it); all threads read it after the unit.
+  unsigned long long* const ctr = unit_next + xcd;
+  if (threadIdx.x == 0) s_next = ub + (int64_t)atomicAdd(ctr, 1ull);
+  __syncthreads();
+  int64_t u = s_next;
   uint32_t found = 0;
   bool bad = false;
-  for (int64_t u = ub + blockIdx.x / kXcds; u < ue; u += per) {
+  while (u < ue) {
+    unsigned long long taken = 0;
+    if (threadIdx.x == 0) taken = atomicAdd(ctr, 1ull);
     const int32_t g = u_g[u];
     const uint32_t L = (uint32_t)Ls[g];
     const int32_t* __restrict__ Tt = T + toff[g];
@@ -526,6 +543,7 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) s_next = ub + (int64_t)taken;
     const int64_t t0 = u_tile[u], t1 = u_tile[u + 1], wb = u_wb[u], we = u_we[u];
     for (int64_t t = t0 + wid; t < t1; t += NW) {
       const int64_t ts = wb + (t - t0) * kStageTile;
@@ -581,6 +599,7 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
       }
     }
     __syncthreads();
+    u = s_next;
   }
   const uint32_t sum = wave_sum(found);
   if (lane == 0 && sum) atomicAdd(out, (unsigned long long)sum);
@@ -745,6 +764,7 @@ static rs_status tc_build_units(const rs_tables* t, hipStream_t st, rs_tc_plan* 
   RS_TRY(p->u_tile.alloc(sizeof(int64_t) * (nu + 1)));
   RS_TRY(p->tile_edge.alloc(sizeof(int32_t) * (size_t)p->num_tiles));
   RS_TRY(p->xcd_units.alloc(sizeof(int64_t) * xsplit.size()));
+  RS_TRY(p->unit_next.alloc(sizeof(unsigned long long) * kTcStagedClasses * kXcds));
   DevBuf de0, de1;
   RS_TRY(de0.alloc(sizeof(int32_t) * nu));
   RS_TRY(de1.alloc(sizeof(int32_t) * nu));
@@ -886,7 +906,7 @@ static rs_status launch_staged(const rs_tables* t, const rs_tc_plan* p, int c, c
       t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->E, p->u_g.as<int32_t>(),
       p->u_wb.as<int64_t>(), p->u_we.as<int64_t>(), p->u_tile.as<int64_t>(),
       p->tile_edge.as<int32_t>(), p->xcd_units.as<int64_t>() + c * (kXcds + 1), words, out,
-      t->d_status());
+      t->d_status(), p->unit_next.as<unsigned long long>() + c * kXcds);
   return RS_OK;
 }
 
@@ -907,6 +927,8 @@ static rs_status launch_staged_b(const rs_tables* t, const rs_tc_plan* p, int c,
 template <int LAYOUT>
 static rs_status tc_count_layout(const rs_tables* t, const rs_tc_plan* p, const int32_t* T,
                                  const int32_t* keys, unsigned long long* out, hipStream_t st) {
+  if (p->class_units[kTcStagedClasses] > p->class_units[0])
+    zero_u64_n<<<1, 64, 0, st>>>(p->unit_next.as<unsigned long long>(), kTcStagedClasses * kXcds);
   if (p->class_units[1] > p->class_units[0])
     RS_TRY((launch_staged_b<LAYOUT, kClassWaves[0]>(t, p, 0, T, keys, out, st)));
   if (p->class_units[2] > p->class_units[1])
