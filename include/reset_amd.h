@@ -312,10 +312,11 @@ typedef struct rs_pairs_opts {
  * schedule 0 (RS_PAIRS_STATIC), wave w takes positions w, w + W, ... (no
  * atomics); schedule k in 1..64, each wave takes runs of k consecutive
  * positions from a shared counter, the reference's work stealing
- * (TC.cu:141-142,176-182, CHUNK = k).  The default is RS_PAIRS_STEAL, runs
- * of 8. */
+ * (TC.cu:141-142,176-182, CHUNK = k).  Runs are shortened to leave at least
+ * 4 runs per wave of the grid; a batch too small for runs of
+ * min(k, 4) is dealt statically.  The default is RS_PAIRS_STEAL, runs of 16. */
 #define RS_PAIRS_STATIC 0
-#define RS_PAIRS_STEAL 8
+#define RS_PAIRS_STEAL 16
 /* Defaults tuned for gfx950 (also the autotuner's starting point). */
 void rs_pairs_default_opts(rs_pairs_opts* o);
 

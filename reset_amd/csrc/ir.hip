@@ -63,15 +63,16 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
   (void)toff32_unused;
   const int lane = __lane_id();
   bool bad = false;
-  // With no more segments than waves (a serving batch), wave w takes segment
-  // w.  With more, work stealing (ir.cu:94-176 takes chunks of queries the
+  // A batch of fewer than 4 runs of kIrRun segments per wave (a serving
+  // batch) is dealt statically: wave w takes segments w, w + W, ...  A
+  // larger one uses work stealing (ir.cu:94-176 takes chunks of queries the
   // same way): a segment's cost varies with its query's term count and
   // survivors, so waves take runs of kIrRun consecutive segments from a
-  // shared counter, the next run one run ahead so that its atomic returns
-  // meanwhile.
+  // shared counter, the next run when the current run's last segment starts,
+  // so that its atomic returns meanwhile.
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  const bool steal = total_segs > nwaves;
+  const bool steal = total_segs >= 4 * kIrRun * nwaves;
   auto take = [&]() -> int64_t {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(next_run, 1ull);
@@ -83,10 +84,10 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
   int32_t r = 0;
   if (steal) {
     q0 = take();
-    q1 = take();
     g = q0;
   }
   while (g < total_segs) {
+    if (steal && r == kIrRun - 1) q1 = take();
     // query owning segment g: largest qi with segoff[qi] <= g
     int64_t lo = 0, hi = Q;
     while (hi - lo > 1) {
@@ -130,13 +131,13 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
       src = dst;
     }
     if (lane == 0) seg_count[g] = cur;
-    if (!steal) break;
-    if (++r < kIrRun) {
+    if (!steal) {
+      g += nwaves;
+    } else if (++r < kIrRun) {
       g = q0 + r;
     } else {
       r = 0;
       q0 = q1;
-      q1 = take();
       g = q0;
     }
   }

@@ -502,8 +502,13 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
 // gets a near-equal mix and no global atomic is needed.  Work stealing
 // (TC.cu:141-142,176-182 takes chunks of edges the same way): lane 0 of a
 // wave takes the next run of K consecutive positions from the control
-// block's counter, one run ahead of the run in use.  Either way the next
-// pair's offsets are fetched before the current pair is processed.
+// block's counter when it hands out the last position of its current run,
+// a pair before the new run is needed.  Runs are cut so that the batch
+// holds at least 4 runs per wave (longer runs leave waves with a whole run
+// where the static deal gives them a few pairs); a batch too small for runs
+// of 4 (or of the requested length, if shorter) is dealt statically.
+// Either way the next pair's offsets are fetched before the current
+// pair is processed.
 template <int B, int U, int CPL, class Src>
 __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     const Src src, int64_t P, const int32_t* __restrict__ order, int32_t* ctl,
@@ -520,29 +525,39 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
   __builtin_amdgcn_wave_barrier();
   const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
-  const int32_t K = cfg.steal;
+  int32_t K = 0;
+  if (cfg.steal > 0) {
+    const int64_t fit = P / (4 * W);
+    K = (int32_t)(cfg.steal < fit ? cfg.steal : fit);
+    if (K < 4 && K < cfg.steal) K = 0;
+  }
   // the counter counts runs: at most P / K + 2W takes, below 2^32
   auto take = [&]() -> int64_t {
     uint32_t v = 0;
     if (lane == 0) v = atomicAdd(reinterpret_cast<uint32_t*>(ctl + kCtlNext), 1u);
     return (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)v) * K;
   };
-  // stealing: the run in use starts at q0 (r positions of it handed out),
-  // the next run at q1
+  // stealing: the run in use starts at q0 (position q0 + r handed out last);
+  // the next run, q1, is taken when the run's last position is handed out
   int64_t q0 = 0, q1 = 0;
   int32_t r = 0;
   auto following = [&](int64_t cur) -> int64_t {
     if (K == 0) return cur + W;
-    if (++r < K) return q0 + r;
-    r = 0;
-    q0 = q1;
-    q1 = take();
-    return q0;
+    int64_t p;
+    if (++r < K) {
+      p = q0 + r;
+    } else {
+      r = 0;
+      q0 = q1;
+      p = q0;
+    }
+    if (r == K - 1) q1 = take();
+    return p;
   };
   int64_t idx;
   if (K > 0) {
     q0 = take();
-    q1 = take();
+    if (K == 1) q1 = take();
     idx = q0;
   } else {
     idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
@@ -795,9 +810,9 @@ void rs_pairs_default_opts(rs_pairs_opts* o) {
   o->num_bins = RS_PAIRS_MAX_BINS;
   o->waves_per_cu = 16;
   o->lds_slots_per_wave = 2048;
-  // work stealing in runs of 8: a pair's cost depends on its size ratio, not
-  // only on its size, so a static deal of ~250 pairs per wave leaves the
-  // slowest wave ~20% behind the mean; runs of 8 keep the shared counter
+  // work stealing in runs of 16: a pair's cost depends on its size ratio,
+  // not only on its size, so a static deal of ~250 pairs per wave leaves the
+  // slowest wave ~20% behind the mean; runs keep the shared counter
   // uncontended (one take per pair stalls on it)
   o->schedule = RS_PAIRS_STEAL;
   for (int i = 0; i < RS_PAIRS_MAX_BINS; ++i) {

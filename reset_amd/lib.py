@@ -17,7 +17,7 @@ REQ_IR, REQ_SSS = 0, 1
 NORMAL, HIERARCHICAL = 0, 1
 BUILD_NORMAL, BUILD_HIERARCHICAL, BUILD_BOTH = 1, 2, 3
 PAIRS_MAX_BINS = 20
-RS_PAIRS_STATIC, RS_PAIRS_STEAL = 0, 8  # rs_pairs_opts.schedule: static, or pairs per work-stealing take
+RS_PAIRS_STATIC, RS_PAIRS_STEAL = 0, 16  # rs_pairs_opts.schedule: static, or pairs per work-stealing take
 
 
 class ResetError(RuntimeError):

@@ -178,31 +178,33 @@ def test_batched_bin_parameters(api, alpha, b, chunk, unroll):
 
 
 def test_schedules(api, oracle_lib):
-    """Static and work-stealing schedules (runs of 1, 3, 8 and 64 pairs) give
-    the constructed sizes on a batch much larger than the grid (every wave
-    takes many pairs); the default stealing schedule also on batches smaller
-    than the grid (most waves take nothing), an empty batch, and in graph
-    replays (the counter is reset by a kernel every launch)."""
+    """Static and work-stealing schedules give the constructed sizes.  With
+    16 waves per CU (4096 waves on 256 CUs) runs are cut to leave 4 runs per
+    wave: on 140k pairs runs of 1, 3 and 8 are kept and runs of 16 and 64 cut
+    to 8; the small batches (an empty one included) are dealt statically;
+    graph replays of a stealing batch match too (the counter is reset by a
+    kernel every launch)."""
     from reset_amd import inputs
-    from reset_amd.lib import RS_PAIRS_STEAL
+    from reset_amd.lib import RS_PAIRS_STEAL, ResetError
 
-    batch = inputs.make_pair_batch(30000, seed=41, hi=8000)
-    for k in (0, 1, 3, 8, 64):
+    P = 140000
+    batch = inputs.make_pair_batch(P, seed=41, hi=2000)
+    for k in (0, 1, 3, 8, 16, 64):
         opts = api.default_pairs_opts()
         opts.schedule = k
-        plan = api.PairsPlan(30000, opts)
+        plan = api.PairsPlan(P, opts)
         got = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"])
         assert torch.equal(got, batch["k"]), k
         plan.close()
     opts = api.default_pairs_opts()
     assert opts.schedule == RS_PAIRS_STEAL
-    plan = api.PairsPlan(30000, opts)
-    for P in (0, 1, 7, 300):
-        sub = plan.run(batch["a_off"][:P + 1].contiguous(), batch["a"], batch["b_off"][:P + 1].contiguous(),
+    plan = api.PairsPlan(P, opts)
+    for n in (0, 1, 7, 300, 20000):
+        sub = plan.run(batch["a_off"][:n + 1].contiguous(), batch["a"], batch["b_off"][:n + 1].contiguous(),
                        batch["b"])
-        assert torch.equal(sub, batch["k"][:P])
+        assert torch.equal(sub, batch["k"][:n]), n
     s = torch.cuda.Stream()
-    sizes = torch.zeros(30000, dtype=torch.int32, device="cuda")
+    sizes = torch.zeros(P, dtype=torch.int32, device="cuda")
     with torch.cuda.stream(s):
         plan.capture(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes, s)
     for rep in range(3):
@@ -213,8 +215,6 @@ def test_schedules(api, oracle_lib):
         assert torch.equal(sizes, batch["k"]), rep
     plan.status()
     plan.close()
-    from reset_amd.lib import ResetError
-
     for k in (-1, 65):
         bad = api.default_pairs_opts()
         bad.schedule = k
