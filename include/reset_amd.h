@@ -163,10 +163,13 @@ rs_status rs_sss_verify_async(const rs_tables* t, int32_t layout, const int32_t*
 
 /* Prefix-filter self-join of a set collection at Jaccard >= p/q: global
  * token order by ascending frequency, prefix candidates with the size filter,
- * verification of every candidate by the batched LDS hash kernel (indexed
- * pairs, rs_pairs_intersect_indexed_async) and the exact rule.  Sets must be
- * ascending with distinct tokens; load_factor/bucket_size are accepted for
- * interface compatibility (the batched kernel sizes its own LDS tables).
+ * verification of every candidate by LDS hash tables (a table per build set
+ * shared by its candidates; the batched pair kernel, indexed form, for build
+ * sets too large for one workgroup's table) and the exact rule.
+ * load_factor (0, 1] and bucket_size (1, 2 or 4) are the verification
+ * tables' hyperparameters (0.25 and 4 are the tuned values; the reference's
+ * 5-slot buckets have no LDS form here and are rejected, RS_EINVAL).  Sets
+ * must hold strictly ascending, non-negative tokens: checked, RS_EINVAL.
  * The result is the sorted list of pairs (i < j) as int64 keys
  * (i << 32 | j), held by the handle. */
 typedef struct rs_sss_result rs_sss_result;
@@ -322,14 +325,17 @@ void rs_pairs_default_opts(rs_pairs_opts* o);
 
 /* The plain form of set_size/intersection.cu:128-181: for each pair, a
  * workspace table of the larger set is filled with -1, built and probed by
- * the smaller set, one launch sequence per pair.  d_workspace must hold
- * bucket_size * calculate_length(max |B|) int32. */
-rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* d_a,
-                                         const int64_t* d_b_off, const int32_t* d_b,
-                                         const int64_t* h_a_off, const int64_t* h_b_off,
-                                         int64_t num_pairs, float load_factor,
-                                         int32_t bucket_size, int32_t* d_workspace,
-                                         int32_t* d_sizes, rs_stream stream);
+ * the smaller set, one launch sequence per pair on `stream`, which is then
+ * synchronized (the reference synchronizes after each pair, :221-243).
+ * d_workspace must hold bucket_size * calculate_length(max |B|) int32.
+ * RS_EOVERFLOW, before any launch, if a set has more keys than its table has
+ * slots (load_factor above 1), and after the launches if a probe walked a
+ * whole (full) table without meeting an EMPTY slot. */
+rs_status rs_pairs_intersect_plain(const int64_t* d_a_off, const int32_t* d_a,
+                                   const int64_t* d_b_off, const int32_t* d_b,
+                                   const int64_t* h_a_off, const int64_t* h_b_off, int64_t num_pairs,
+                                   float load_factor, int32_t bucket_size, int32_t* d_workspace,
+                                   int32_t* d_sizes, rs_stream stream);
 
 /* The batched form: one persistent, size-binned launch whose waves take pairs
  * statically or by work stealing (opts.schedule).  A

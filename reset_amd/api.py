@@ -309,8 +309,9 @@ class CompetitorTables:
             pass
 
 
-def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.2, bucket_size=5, stream=None):
-    """Prefix-filter self-join at Jaccard >= p/q with hash verification.
+def sss_selfjoin(offsets, elements, p=4, q=5, load_factor=0.25, bucket_size=4, stream=None):
+    """Prefix-filter self-join at Jaccard >= p/q with hash verification;
+    load_factor / bucket_size (1, 2 or 4) set the verification tables.
     -> dict(pairs: int64 tensor of i<<32|j ascending (device), candidates,
             verify_probes, ms: [order, candidates, verify])"""
     _need_cuda(offsets, elements)
@@ -449,11 +450,10 @@ def pairs_intersect_plain(a_off, a, b_off, b, load_factor=0.2, bucket_size=5, st
     L = calculate_length(maxb, load_factor, bucket_size)
     ws = torch.empty(max(L * bucket_size, 1), dtype=torch.int32, device=a.device)
     sizes = torch.empty(max(P, 1), dtype=torch.int32, device=a.device)
-    check(load().rs_pairs_intersect_plain_async(ptr(a_off), ptr(a), ptr(b_off), ptr(b),
-                                                C.c_void_p(h_a.data_ptr()), C.c_void_p(h_b.data_ptr()), P,
-                                                float(load_factor), int(bucket_size), ptr(ws), ptr(sizes),
-                                                stream_handle(stream)))
-    torch.cuda.synchronize()
+    check(load().rs_pairs_intersect_plain(ptr(a_off), ptr(a), ptr(b_off), ptr(b),
+                                          C.c_void_p(h_a.data_ptr()), C.c_void_p(h_b.data_ptr()), P,
+                                          float(load_factor), int(bucket_size), ptr(ws), ptr(sizes),
+                                          stream_handle(stream)))
     return sizes[:P]
 
 

@@ -49,10 +49,10 @@ int main(int argc, char** argv) {
     DeviceArray<int> da(a), db(b), ws((size_t)bs * rs_calculate_length(big, lf, bs)),
         sizes(1);
     const float plain_ms = time_ms([&] {
-      check(rs_pairs_intersect_plain_async(da_off.data(), da.data(), db_off.data(), db.data(),
-                                           a_off.data(), b_off.data(), 1, lf, bs, ws.data(),
-                                           sizes.data(), nullptr),
-            "rs_pairs_intersect_plain_async");
+      check(rs_pairs_intersect_plain(da_off.data(), da.data(), db_off.data(), db.data(),
+                                     a_off.data(), b_off.data(), 1, lf, bs, ws.data(), sizes.data(),
+                                     nullptr),
+            "rs_pairs_intersect_plain");
     });
     const int plain = sizes.to_host()[0];
     std::printf("Normal Intersection Time: %f ms\n", plain_ms);

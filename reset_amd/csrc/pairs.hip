@@ -393,7 +393,7 @@ template <int B, int U>
 __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict__ S, int32_t ns,
                                                       const int32_t* __restrict__ G, int32_t ng,
                                                       int32_t* tab, int32_t* ring, float alpha,
-                                                      int32_t chunk, bool& bad) {
+                                                      int32_t chunk, bool& bad, bool& unordered) {
   constexpr int32_t kBlock = 4 * U * kWave;
   const int lane = __lane_id();
   // the bin's table: calculate_length(chunk) buckets (fits the wave's region)
@@ -426,6 +426,12 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
   int32_t cb = 0, ce = s0 + c < ns ? s0 + c : ns;
   bool last = ce >= ns;
   int32_t spos[4];
+  // The current chunk answers for the values [lo, top]: lo is one above the
+  // previous chunk's last element (0 for the first chunk), top its own last
+  // element (all values for the last chunk).  A build key outside [lo, top]
+  // or a probe key below lo would be answered by the wrong chunk: that is
+  // input out of ascending order, flagged (kErrBadInput) rather than counted.
+  uint32_t lo = 0u, top = 0u;
   auto build = [&]() {
     // stage whatever the chunk needs that is not staged yet
     const int32_t gh = group_of(ce - 1);
@@ -435,22 +441,25 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       ++sc;
     }
     __builtin_amdgcn_wave_barrier();
+    const uint32_t last_el = (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
+    const uint32_t hi = last ? 0xFFFFFFFFu : last_el;
     bool moved = false;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int32_t idx = cb + r * kWave + lane;
       spos[r] = -1;
       if (idx < ce) {
-        spos[r] = lds_insert<B>(tab, L, ring_at(idx), moved);
+        const int32_t v = ring_at(idx);
+        unordered |= (uint32_t)v < lo || (uint32_t)v > hi;
+        spos[r] = lds_insert<B>(tab, L, v, moved);
         bad |= spos[r] < 0;
       }
     }
     __builtin_amdgcn_wave_barrier();
+    top = hi;
     return __any(moved);
   };
   bool displaced = build();
-  // largest value the current table answers for
-  uint32_t top = last ? 0xFFFFFFFFu : (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
   int32_t pos = block_origin(G);
   GBlock<U> cur;
   load_block<U>(G, pos, 0, ng, cur);
@@ -468,6 +477,7 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       for (int i = 0; i < 4 * U; ++i) {
         const bool below = (uint32_t)cur.x[i] <= top;
         in[i] = cur.act[i] && below;
+        unordered |= in[i] && (uint32_t)cur.x[i] < lo;
         cur.act[i] = cur.act[i] && !below;
         rest |= cur.act[i];
       }
@@ -475,11 +485,11 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       if (last || !__any(rest)) break;
       // the block reaches past this chunk: rebuild from the next one
       clear_chunk<4>(tab, spos);
+      lo = top + 1u;
       cb = ce;
       ce = ns - cb < c ? ns : cb + c;  // (cb is on the chunk grid from here on)
       last = ce >= ns;
       displaced = build();
-      top = last ? 0xFFFFFFFFu : (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
     }
     if (!more) break;
     // stage the group loaded above if it is still the next one and its ring
@@ -563,7 +573,7 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
   }
   int64_t nidx = following(idx);
-  bool bad = false;
+  bool bad = false, unordered = false;
   int32_t i = 0;
   const int32_t* A = nullptr;
   const int32_t* Bp = nullptr;
@@ -606,8 +616,10 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
         hits = wide ? pair_resident_wave<B, U, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad)
                     : pair_resident_wave<B, 1, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad);
       else
-        hits = wide ? pair_chunked_wave<B, U>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad)
-                    : pair_chunked_wave<B, 1>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad);
+        hits = wide ? pair_chunked_wave<B, U>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad,
+                                              unordered)
+                    : pair_chunked_wave<B, 1>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad,
+                                              unordered);
     }
     const uint32_t total = wave_sum(hits);
     if (lane == 0) sizes[i] = (int32_t)total;
@@ -620,6 +632,7 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     nidx = nnidx;
   }
   if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
+  if (__any(unordered) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrBadInput);
 }
 
 // ---------------------------------------------------------------- plain form
@@ -662,6 +675,8 @@ __global__ void plain_probe_kernel(const int32_t* __restrict__ A, int64_t na,
   if (__lane_id() == 0 && s) atomicAdd(size_slot, (int32_t)s);
   if (__any(r < 0) && __lane_id() == 0) atomicOr(err, kErrProbeBound);
 }
+
+__global__ void set_word_kernel(int32_t* p, int32_t v) { *p = v; }
 
 }  // namespace rs
 
@@ -960,28 +975,54 @@ rs_status rs_pairs_plan_status(rs_pairs_plan* p, rs_stream stream) {
   if (err) {
     RS_HIP(hipMemsetAsync(p->ctl.as<int32_t>() + kCtlStatus, 0, sizeof(int32_t), st));
     RS_HIP(hipStreamSynchronize(st));
+    if (err & kErrBadInput) {
+      set_error("pairs: a pair whose smaller set exceeds its bin's chunk has a set that is not "
+                "strictly ascending (the chunked path needs ascending sets)");
+      return RS_EINVAL;
+    }
     set_error("pairs: an LDS table had no free slot or a probe hit its bound");
     return RS_EOVERFLOW;
   }
   return RS_OK;
 }
 
-rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* d_a,
-                                         const int64_t* d_b_off, const int32_t* d_b,
-                                         const int64_t* h_a_off, const int64_t* h_b_off,
-                                         int64_t num_pairs, float load_factor,
-                                         int32_t bucket_size, int32_t* d_workspace,
-                                         int32_t* d_sizes, rs_stream stream) {
+rs_status rs_pairs_intersect_plain(const int64_t* d_a_off, const int32_t* d_a,
+                                   const int64_t* d_b_off, const int32_t* d_b,
+                                   const int64_t* h_a_off, const int64_t* h_b_off, int64_t num_pairs,
+                                   float load_factor, int32_t bucket_size, int32_t* d_workspace,
+                                   int32_t* d_sizes, rs_stream stream) {
   RS_REQUIRE(d_a_off && d_b_off && h_a_off && h_b_off && d_workspace && (d_sizes || !num_pairs),
-             "rs_pairs_intersect_plain_async: null argument");
+             "rs_pairs_intersect_plain: null argument");
   RS_REQUIRE(load_factor > 0.f && bucket_size > 0, "plain pairs: bad alpha/b");
+  RS_REQUIRE(num_pairs >= 0, "plain pairs: negative pair count");
   (void)d_a_off;
   (void)d_b_off;
+  // A table of L buckets x b slots takes at most L*b keys: a larger set
+  // (load_factor above 1) cannot be built, and the reference's unbounded
+  // insert would spin (intersection.cu:128-152).  Rejected before any launch.
+  for (int64_t i = 0; i < num_pairs; ++i) {
+    const int64_t na = h_a_off[i + 1] - h_a_off[i], nb = h_b_off[i + 1] - h_b_off[i];
+    RS_REQUIRE(na >= 0 && nb >= 0 && na < (1ll << 31) && nb < (1ll << 31),
+               "plain pairs: pair %lld has a bad offset range", (long long)i);
+    const int64_t nB = na > nb ? na : nb;
+    const int32_t L = calc_length((int32_t)nB, load_factor, bucket_size);
+    RS_REQUIRE(L > 0, "plain pairs: set too large");
+    if (nB > (int64_t)L * bucket_size) {
+      set_error("plain pairs: pair %lld: %lld keys exceed the %d x %d slots of its table "
+                "(load factor %g)", (long long)i, (long long)nB, L, bucket_size, (double)load_factor);
+      return RS_EOVERFLOW;
+    }
+  }
   hipStream_t st = as_stream(stream);
-  // error word lives after the sizes in a tiny side buffer
-  static thread_local DevBuf err_buf;
-  if (!err_buf.p) RS_TRY(err_buf.alloc(16));
-  int32_t* err = err_buf.as<int32_t>();
+  // the call's error word: one per host thread and device, cleared by a
+  // kernel here and read back after the last pair
+  int dev = 0;
+  RS_HIP(hipGetDevice(&dev));
+  RS_REQUIRE(dev >= 0 && dev < 64, "plain pairs: device ordinal %d out of range", dev);
+  static thread_local DevBuf err_buf[64];
+  if (!err_buf[dev].p) RS_TRY(err_buf[dev].alloc(16));
+  int32_t* err = err_buf[dev].as<int32_t>();
+  set_word_kernel<<<1, 1, 0, st>>>(err, 0);
   for (int64_t i = 0; i < num_pairs; ++i) {
     const int64_t a0 = h_a_off[i], na = h_a_off[i + 1] - a0;
     const int64_t b0 = h_b_off[i], nb = h_b_off[i + 1] - b0;
@@ -997,7 +1038,6 @@ rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* 
       nB = tn;
     }
     const int32_t L = calc_length((int32_t)nB, load_factor, bucket_size);
-    RS_REQUIRE(L > 0, "plain pairs: set too large");
     RS_HIP(hipMemsetAsync(d_workspace, 0xFF, sizeof(int32_t) * (size_t)L * bucket_size, st));
     const unsigned gb = (unsigned)((nB + 255) / 256);
     plain_build_kernel<<<gb ? gb : 1, 256, 0, st>>>(B, nB, d_workspace, (uint32_t)L, bucket_size,
@@ -1007,6 +1047,13 @@ rs_status rs_pairs_intersect_plain_async(const int64_t* d_a_off, const int32_t* 
           A, nA, d_workspace, (uint32_t)L, bucket_size, d_sizes + i, err);
   }
   RS_HIP(hipGetLastError());
+  int32_t h_err = 0;
+  RS_HIP(hipMemcpyAsync(&h_err, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  if (h_err) {
+    set_error("plain pairs: a table had no free slot or a probe walked the whole table");
+    return RS_EOVERFLOW;
+  }
   return RS_OK;
 }
 

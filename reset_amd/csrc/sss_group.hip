@@ -4,7 +4,8 @@
 // token is a candidate of hundreds of others, so the pairs of one u form long
 // runs.  A run (cut into units of at most kGroupCands candidates) is verified
 // by one workgroup: it inserts S_u into an LDS table once (bucket-major,
-// kGroupB-slot buckets, load factor kGroupAlpha), then its waves stream the
+// B-slot buckets with B in {1, 2, 4}, load factor alpha: the table's
+// hyperparameters, 1/4 and 4 by default), then its waves stream the
 // partner sets S_v through it, one candidate per wave at a time.  Per pair
 // the cost is the streamed set alone; the build is paid once per unit.
 //
@@ -23,22 +24,31 @@ namespace rs {
 
 constexpr int kGroupBlock = 256;
 constexpr int kGroupWaves = kGroupBlock / kWave;
-constexpr int kGroupB = 4;
-constexpr float kGroupAlpha = 0.25f;
 constexpr int64_t kGroupCands = 1024;
 
-// Largest build set the grouped path takes: calculate_length(2047, 1/4, 4) =
-// 2048 buckets x 4 slots = 32 KiB of LDS per workgroup.
-constexpr int32_t kGroupMaxBuild = 2047;
-constexpr int32_t kGroupSlots = 2048 * kGroupB;
+// LDS of one workgroup's table: 8192 int32 slots (32 KiB), so 4 workgroups
+// share a CU.  The largest build set the grouped path takes is the largest
+// size whose table fits (group_max_build): 2047 at alpha 1/4, b 4.
+constexpr int32_t kGroupSlots = 8192;
 
+template <int B>
 __device__ __forceinline__ void group_bucket(const int32_t* __restrict__ tab, uint32_t j, int32_t x,
                                              bool& hit, bool& full) {
-  const int4 v = *reinterpret_cast<const int4*>(tab + j * kGroupB);
-  uint32_t m = min(min((uint32_t)(v.x ^ x), (uint32_t)(v.y ^ x)),
-                   min((uint32_t)(v.z ^ x), (uint32_t)(v.w ^ x)));
-  hit = m == 0u;
-  full = v.w != kEmpty;
+  if constexpr (B == 4) {
+    const int4 v = *reinterpret_cast<const int4*>(tab + j * 4);
+    const uint32_t m = min(min((uint32_t)(v.x ^ x), (uint32_t)(v.y ^ x)),
+                           min((uint32_t)(v.z ^ x), (uint32_t)(v.w ^ x)));
+    hit = m == 0u;
+    full = v.w != kEmpty;
+  } else if constexpr (B == 2) {
+    const int2 v = *reinterpret_cast<const int2*>(tab + j * 2);
+    hit = v.x == x || v.y == x;
+    full = v.y != kEmpty;
+  } else {
+    const int32_t v = tab[j];
+    hit = v == x;
+    full = v != kEmpty;
+  }
 }
 
 // unit_start[k] = first candidate of unit k: a candidate opens a unit when
@@ -73,8 +83,9 @@ struct PartnerBlock {
   bool valid;
 };
 
+template <int B>
 __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
-    const int32_t* __restrict__ a_off, const int32_t* __restrict__ a_el,
+    float alpha, const int32_t* __restrict__ a_off, const int32_t* __restrict__ a_el,
     const int32_t* __restrict__ b_off, const int32_t* __restrict__ b_el,
     const int32_t* __restrict__ cu, const int32_t* __restrict__ cv,
     const int64_t* __restrict__ unit_start, int64_t nunits, int64_t ncand,
@@ -118,14 +129,14 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
     // deep inside it, where a wait would also cover the block loads)
     __asm__ volatile("" : : "v"(mb0[0]), "v"(mb0[1]), "v"(mb0[2]), "v"(mb0[3]), "v"(mnb[0]),
                      "v"(mnb[1]), "v"(mnb[2]), "v"(mnb[3]));
-    const uint32_t L = (uint32_t)calc_length(na, kGroupAlpha, kGroupB);
-    const uint32_t nslots = L * kGroupB;  // <= kGroupSlots (build sets are <= kGroupMaxBuild)
+    const uint32_t L = (uint32_t)calc_length(na, alpha, B);
+    const uint32_t nslots = L * B;  // <= kGroupSlots (build sets are <= group_max_build)
     for (uint32_t s = threadIdx.x; s < nslots / 4; s += kGroupBlock)
       *reinterpret_cast<int4*>(tab + 4 * s) = int4{kEmpty, kEmpty, kEmpty, kEmpty};
     __syncthreads();
     for (int32_t t = threadIdx.x; t < na; t += kGroupBlock) {
       const int32_t x = a_el[a0 + t];
-      uint32_t slot = ((uint32_t)x & (L - 1u)) * kGroupB;
+      uint32_t slot = ((uint32_t)x & (L - 1u)) * B;
       uint32_t step = 0;
       for (; step < nslots; ++step) {
         if (atomicCAS(&tab[slot], kEmpty, x) == kEmpty) break;
@@ -188,7 +199,7 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
           act[q] = e >= 0 && e < b.nb;
           j[q] = (uint32_t)x[q] & (L - 1u);
           bool hit, full;
-          group_bucket(tab, j[q], x[q], hit, full);
+          group_bucket<B>(tab, j[q], x[q], hit, full);
           hits += act[q] && hit;
           pend[q] = act[q] && !hit && full;
           any |= pend[q];
@@ -201,7 +212,7 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
             for (uint32_t step = 1; step < L && __any(p); ++step) {
               jj = (jj + 1u) & (L - 1u);
               bool hit, full;
-              group_bucket(tab, jj, x[q], hit, full);
+              group_bucket<B>(tab, jj, x[q], hit, full);
               hits += p && hit;
               p = p && !hit && full;
             }
@@ -244,14 +255,27 @@ __global__ void __launch_bounds__(kGroupBlock) group_verify_kernel(
 __global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
 
 // Which candidates the grouped path takes: build set no larger than
-// kGroupMaxBuild.  sel[i] = 1 for those.
+// max_build.  sel[i] = 1 for those.
 __global__ void group_select_kernel(const int32_t* __restrict__ a_off, const int32_t* __restrict__ cu,
-                                    int64_t n, int32_t* __restrict__ sel) {
+                                    int64_t n, int32_t max_build, int32_t* __restrict__ sel) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t u = cu[i];
-    sel[i] = a_off[u + 1] - a_off[u] <= kGroupMaxBuild;
+    sel[i] = a_off[u + 1] - a_off[u] <= max_build;
   }
+}
+
+int32_t group_max_build(float alpha, int32_t b) {
+  // calculate_length is non-decreasing in the size: bisect for the largest
+  // size whose table has at most kGroupSlots slots
+  int32_t lo = 0, hi = kGroupSlots;  // a table of >= 8192 keys never fits
+  while (hi - lo > 1) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    const int32_t L = calc_length(mid, alpha, b);
+    if (L > 0 && (int64_t)L * b <= kGroupSlots) lo = mid; else hi = mid;
+  }
+  const int32_t L0 = calc_length(lo, alpha, b);
+  return L0 > 0 && (int64_t)L0 * b <= kGroupSlots ? lo : -1;
 }
 
 #define RS_ROC_G(expr)                                                \
@@ -263,9 +287,10 @@ __global__ void group_select_kernel(const int32_t* __restrict__ a_off, const int
     }                                                                 \
   } while (0)
 
-rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
-                       const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t n,
-                       int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st) {
+rs_status group_verify(float alpha, int32_t b, const int32_t* a_off, const int32_t* a_el,
+                       const int32_t* b_off, const int32_t* b_el, const int32_t* cu, const int32_t* cv,
+                       int64_t n, int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st) {
+  RS_REQUIRE(b == 1 || b == 2 || b == 4, "sss: grouped table bucket size %d (needs 1, 2 or 4)", b);
   if (n == 0) return RS_OK;
   DevBuf flags, starts, nsel;
   RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)n));
@@ -287,17 +312,17 @@ rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t*
   // select count read above)
   auto* next_unit = reinterpret_cast<unsigned long long*>(nsel.as<size_t>() + 1);
   zero_u64_kernel<<<1, 1, 0, st>>>(next_unit);
-  group_verify_kernel<<<grid, kGroupBlock, 0, st>>>(a_off, a_el, b_off, b_el, cu, cv,
-                                                    starts.as<int64_t>(), (int64_t)nunits, n, sizes,
-                                                    d_err, next_unit);
+  auto kern = b == 4 ? group_verify_kernel<4> : b == 2 ? group_verify_kernel<2> : group_verify_kernel<1>;
+  kern<<<grid, kGroupBlock, 0, st>>>(alpha, a_off, a_el, b_off, b_el, cu, cv, starts.as<int64_t>(),
+                                     (int64_t)nunits, n, sizes, d_err, next_unit);
   RS_HIP(hipGetLastError());
   RS_HIP(hipStreamSynchronize(st));  // flags/starts are released on return
   return RS_OK;
 }
 
-rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t* sel,
-                          hipStream_t st) {
-  if (n > 0) group_select_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(a_off, cu, n, sel);
+rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t max_build,
+                          int32_t* sel, hipStream_t st) {
+  if (n > 0) group_select_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(a_off, cu, n, max_build, sel);
   RS_HIP(hipGetLastError());
   return RS_OK;
 }

@@ -324,12 +324,55 @@ static rs_status unique_sorted(DevBuf& keys, int64_t C, DevBuf& tmp, hipStream_t
                                int64_t* U_out);
 static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
                                    const int32_t* b_el, DevBuf& uniq, int64_t U, int32_t p, int32_t q,
-                                   DevBuf& tmp, hipStream_t st, rs_sss_result* R);
+                                   float alpha, int32_t bsz, DevBuf& tmp, hipStream_t st,
+                                   rs_sss_result* R);
+
+// One wave per set: flags a set whose tokens are not strictly ascending
+// (a repeated token would count twice in |S| and in the intersection, and
+// the chunked pair kernel needs ascending sets) or that holds a negative id.
+__global__ void ascending_check_kernel(const int32_t* __restrict__ off, int64_t n,
+                                       const int32_t* __restrict__ el, int32_t* err) {
+  const int lane = __lane_id();
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  bool bad = false;
+  for (int64_t i = wave; i < n; i += nwaves) {
+    const int32_t s = off[i], e = off[i + 1];
+    bad |= e < s;
+    for (int32_t j = s + lane; j < e; j += kWave) {
+      const int32_t x = el[j];
+      bad |= x < 0 || (j + 1 < e && el[j + 1] <= x);
+    }
+  }
+  if (__any(bad) && lane == 0) atomicOr(err, kErrBadInput);
+}
+
+// RS_EINVAL unless every set of the CSR is strictly ascending and
+// non-negative; synchronizes st.
+static rs_status check_ascending(const int32_t* d_off, int64_t n, const int32_t* d_el, const char* who,
+                                 hipStream_t st) {
+  if (n == 0) return RS_OK;
+  DevBuf err;
+  RS_TRY(err.alloc(16));
+  RS_HIP(hipMemsetAsync(err.p, 0, 16, st));
+  ascending_check_kernel<<<grid_for(n * kWave, 256, 8), 256, 0, st>>>(d_off, n, d_el, err.as<int32_t>());
+  int32_t h = 0;
+  RS_HIP(hipMemcpyAsync(&h, err.p, sizeof(h), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(h == 0, "%s: every set must hold strictly ascending, non-negative tokens", who);
+  return RS_OK;
+}
 
 static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,
                           int32_t q, float alpha, int32_t bsz, hipStream_t st, rs_sss_result* R) {
   RS_REQUIRE(n >= 0 && n < (1ll << 21), "sss self-join: %lld sets exceed 2^21", (long long)n);
   RS_REQUIRE(p > 0 && q >= p, "sss self-join: threshold p/q must be in (0, 1]");
+  RS_REQUIRE(alpha > 0.f && alpha <= 1.f && (bsz == 1 || bsz == 2 || bsz == 4),
+             "sss self-join: the verification tables need load_factor in (0, 1] and bucket_size "
+             "1, 2 or 4 (got %g, %d)", (double)alpha, bsz);
+  RS_REQUIRE(group_max_build(alpha, bsz) >= 0,
+             "sss self-join: no set fits the LDS table at load_factor %g", (double)alpha);
+  RS_TRY(check_ascending(d_off, n, d_el, "sss self-join", st));
   hipEvent_t ev[4];
   for (auto& e : ev) RS_HIP(hipEventCreate(&e));
   struct EvGuard {
@@ -391,7 +434,7 @@ static rs_status selfjoin(const int32_t* d_off, int64_t n, const int32_t* d_el, 
   RS_HIP(hipEventRecord(ev[2], st));
   // 6. verification: every candidate's |S_i cap S_j| by the batched LDS
   //    hash kernel (indexed pairs), then the exact rule + compaction
-  RS_TRY(verify_candidates(d_off, d_el, d_off, d_el, uniq, U, p, q, tmp, st, R));
+  RS_TRY(verify_candidates(d_off, d_el, d_off, d_el, uniq, U, p, q, alpha, bsz, tmp, st, R));
   RS_HIP(hipEventRecord(ev[3], st));
   RS_HIP(hipEventSynchronize(ev[3]));
   for (int i = 0; i < 3; ++i) RS_HIP(hipEventElapsedTime(&R->ms[i], ev[i], ev[i + 1]));
@@ -543,16 +586,18 @@ static rs_status select_indices(const int32_t* flags, int64_t n, DevBuf& out, in
 // cu (sss_group.hip); the rest go through the batched pair kernel.
 static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
                                  const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t U,
-                                 int32_t* sizes, DevBuf& tmp, hipStream_t st) {
+                                 float alpha, int32_t bsz, int32_t* sizes, DevBuf& tmp,
+                                 hipStream_t st) {
   DevBuf sel, err, idx_g;
   RS_TRY(sel.alloc(sizeof(int32_t) * (size_t)U));
   RS_TRY(err.alloc(16));
   RS_HIP(hipMemsetAsync(err.p, 0, 16, st));
-  RS_TRY(group_partition(a_off, cu, U, sel.as<int32_t>(), st));
+  RS_TRY(group_partition(a_off, cu, U, group_max_build(alpha, bsz), sel.as<int32_t>(), st));
   int64_t G = 0;
   RS_TRY(select_indices(sel.as<int32_t>(), U, idx_g, &G, tmp, st));
   if (G == U) {
-    RS_TRY(group_verify(a_off, a_el, b_off, b_el, cu, cv, U, sizes, err.as<int32_t>(), tmp, st));
+    RS_TRY(group_verify(alpha, bsz, a_off, a_el, b_off, b_el, cu, cv, U, sizes, err.as<int32_t>(), tmp,
+                        st));
   } else {
     if (G > 0) {
       DevBuf gu, gv, gs;
@@ -562,7 +607,7 @@ static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, cons
       const int g = grid_for(G, 256, 8);
       gather_i32_kernel<<<g, 256, 0, st>>>(cu, idx_g.as<int64_t>(), G, gu.as<int32_t>());
       gather_i32_kernel<<<g, 256, 0, st>>>(cv, idx_g.as<int64_t>(), G, gv.as<int32_t>());
-      RS_TRY(group_verify(a_off, a_el, b_off, b_el, gu.as<int32_t>(), gv.as<int32_t>(), G,
+      RS_TRY(group_verify(alpha, bsz, a_off, a_el, b_off, b_el, gu.as<int32_t>(), gv.as<int32_t>(), G,
                           gs.as<int32_t>(), err.as<int32_t>(), tmp, st));
       scatter_i32_kernel<<<g, 256, 0, st>>>(gs.as<int32_t>(), idx_g.as<int64_t>(), G, sizes);
     }
@@ -579,8 +624,20 @@ static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, cons
     const int g = grid_for(R, 256, 8);
     gather_i32_kernel<<<g, 256, 0, st>>>(cu, idx_r.as<int64_t>(), R, ru.as<int32_t>());
     gather_i32_kernel<<<g, 256, 0, st>>>(cv, idx_r.as<int64_t>(), R, rv.as<int32_t>());
+    // the batched pair kernel's bins at the same (alpha, b): the largest
+    // chunk whose table fits a wave's LDS region
+    rs_pairs_opts o;
+    rs_pairs_default_opts(&o);
+    int32_t chunk = 0;
+    for (int32_t c = 64; c <= 1024; c += 64) {
+      const int32_t L = calc_length(c, alpha, bsz);
+      if (L > 0 && (int64_t)L * bsz <= o.lds_slots_per_wave) chunk = c;
+    }
+    RS_REQUIRE(chunk > 0, "sss: no 64-element chunk fits a pair table at load_factor %g", (double)alpha);
+    if (alpha != kVerifyAlpha || bsz != kVerifyB)
+      for (int i = 0; i < o.num_bins; ++i) o.bins[i] = rs_pairs_bin{alpha, bsz, chunk, o.bins[i].unroll};
     rs_pairs_plan* plan = nullptr;
-    RS_TRY(rs_pairs_plan_create(R, nullptr, &plan));
+    RS_TRY(rs_pairs_plan_create(R, &o, &plan));
     struct PGuard {
       rs_pairs_plan* p;
       ~PGuard() { rs_pairs_plan_free(p); }
@@ -604,7 +661,8 @@ static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, cons
 // sizes by the batched LDS kernel, the exact rule, compaction into R.
 static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
                                    const int32_t* b_el, DevBuf& uniq, int64_t U, int32_t p, int32_t q,
-                                   DevBuf& tmp, hipStream_t st, rs_sss_result* R) {
+                                   float alpha, int32_t bsz, DevBuf& tmp, hipStream_t st,
+                                   rs_sss_result* R) {
   if (U == 0) {
     RS_TRY(R->pairs.alloc(16));
     R->num_pairs = 0;
@@ -619,7 +677,7 @@ static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, co
   RS_HIP(hipMemsetAsync(work.p, 0, 16, st));
   split_pairs_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(uniq.as<uint64_t>(), U, cu.as<int32_t>(),
                                                          cv.as<int32_t>());
-  RS_TRY(sizes_by_groups(a_off, a_el, b_off, b_el, cu.as<int32_t>(), cv.as<int32_t>(), U,
+  RS_TRY(sizes_by_groups(a_off, a_el, b_off, b_el, cu.as<int32_t>(), cv.as<int32_t>(), U, alpha, bsz,
                          sizes.as<int32_t>(), tmp, st));
   jaccard_flags_kernel<<<grid_for(U, 256, 8), 256, 0, st>>>(a_off, b_off, cu.as<int32_t>(), cv.as<int32_t>(),
                                                            sizes.as<int32_t>(), U, p, q,
@@ -648,6 +706,7 @@ static rs_status index_build(const int32_t* d_off, int64_t n, const int32_t* d_e
                              int32_t q, hipStream_t st, rs_sss_index* X) {
   RS_REQUIRE(n >= 0 && n < (1ll << 21), "rs_sss_index_build: %lld sets exceed 2^21", (long long)n);
   RS_REQUIRE(p > 0 && q >= p, "rs_sss_index_build: threshold p/q must be in (0, 1]");
+  RS_TRY(check_ascending(d_off, n, d_el, "rs_sss_index_build", st));
   int32_t E32 = 0;
   if (n > 0) {
     RS_HIP(hipMemcpyAsync(&E32, d_off + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -748,7 +807,8 @@ static rs_status lookup(const rs_sss_index* X, const int32_t* d_qoff, int64_t Q,
   }
   R->num_candidates = U;
   RS_HIP(hipEventRecord(ev[2], st));
-  RS_TRY(verify_candidates(d_qoff, d_qtok, X->off, X->el, uniq, U, X->p, X->q, tmp, st, R));
+  RS_TRY(verify_candidates(d_qoff, d_qtok, X->off, X->el, uniq, U, X->p, X->q, kVerifyAlpha, kVerifyB,
+                           tmp, st, R));
   RS_HIP(hipEventRecord(ev[3], st));
   RS_HIP(hipEventSynchronize(ev[3]));
   for (int i = 0; i < 3; ++i) RS_HIP(hipEventElapsedTime(&R->ms[i], ev[i], ev[i + 1]));

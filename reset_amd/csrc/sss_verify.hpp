@@ -14,12 +14,20 @@ rs_status sss_verify_enqueue(const rs_tables* t, int32_t layout, const int32_t* 
                              hipStream_t st);
 
 // Grouped verification (sss_group.hip): sizes[i] = |A_{cu[i]} cap B_{cv[i]}|
-// for candidates sorted by cu whose build set A_{cu[i]} has at most 2047
-// elements (group_partition flags those: sel[i] = 1).  Synchronizes st.
-rs_status group_verify(const int32_t* a_off, const int32_t* a_el, const int32_t* b_off,
-                       const int32_t* b_el, const int32_t* cu, const int32_t* cv, int64_t n,
-                       int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st);
-rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t* sel,
-                          hipStream_t st);
+// for candidates sorted by cu whose build set A_{cu[i]} fits the grouped
+// LDS table at load factor alpha and bucket size b in {1, 2, 4}: at most
+// group_max_build(alpha, b) elements (-1: no size fits; 2047 at 1/4, 4).
+// group_partition flags those candidates: sel[i] = 1.  Synchronizes st.
+int32_t group_max_build(float alpha, int32_t b);
+rs_status group_verify(float alpha, int32_t b, const int32_t* a_off, const int32_t* a_el,
+                       const int32_t* b_off, const int32_t* b_el, const int32_t* cu, const int32_t* cv,
+                       int64_t n, int32_t* sizes, int32_t* d_err, DevBuf& tmp, hipStream_t st);
+rs_status group_partition(const int32_t* a_off, const int32_t* cu, int64_t n, int32_t max_build,
+                          int32_t* sel, hipStream_t st);
+
+// The verification tables' default hyperparameters (the grouped LDS table and
+// the batched pair kernel's bins).
+constexpr float kVerifyAlpha = 0.25f;
+constexpr int32_t kVerifyB = 4;
 
 }  // namespace rs

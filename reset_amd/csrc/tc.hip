@@ -77,9 +77,12 @@ constexpr size_t kLdsBytes = 160 * 1024;
 
 __host__ __device__ inline int tc_class_of(uint32_t L) { return L <= 1024 ? 0 : (L <= 2048 ? 1 : 2); }
 
-// LDS bytes of a class-c workgroup staging tables of length <= L.
+// LDS bytes of a class-c workgroup staging tables of length <= L: the
+// table, each wave's marker row, and 16 bytes for the next-unit word (the
+// kernel keeps no static LDS, so this is its whole LDS).
+constexpr size_t kStageWordBytes = 16;
 inline size_t stage_lds_bytes(int c, int64_t L, int32_t b) {
-  return (size_t)L * (size_t)b * 4 + (size_t)kClassWaves[c] * kWave * kStageU * 4;
+  return (size_t)L * (size_t)b * 4 + (size_t)kClassWaves[c] * kWave * kStageU * 4 + kStageWordBytes;
 }
 
 // Largest length that may be staged with bucket size b (0 = none).
@@ -500,11 +503,13 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
     unsigned long long* __restrict__ out, int32_t* __restrict__ status,
     unsigned long long* __restrict__ unit_next) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_tc[];
-  __shared__ int64_t s_next;
   int32_t* const tab = lds_tc;
   const int lane = __lane_id();
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int32_t* const marks = lds_tc + table_words + wid * (kWave * U);
+  // the next-unit word sits after every wave's marker row (table_words is a
+  // multiple of 8 and each row holds 64*U words, so it is 8-byte aligned)
+  int64_t& s_next = *reinterpret_cast<int64_t*>(lds_tc + table_words + NW * (kWave * U));
 #pragma unroll
   for (int u = 0; u < U; ++u) marks[u * kWave + lane] = -1;
   int32_t tag = 0;

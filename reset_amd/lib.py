@@ -99,7 +99,7 @@ SIGNATURES = {
     "rs_server_stats": (_i32, [_p, _p, _p, _p]),
     "rs_server_free": (None, [_p]),
     "rs_pairs_default_opts": (None, [C.POINTER(rs_pairs_opts)]),
-    "rs_pairs_intersect_plain_async": (_i32, [_p, _p, _p, _p, _p, _p, _i64, _f32, _i32, _p, _p, _p]),
+    "rs_pairs_intersect_plain": (_i32, [_p, _p, _p, _p, _p, _p, _i64, _f32, _i32, _p, _p, _p]),
     "rs_pairs_plan_create": (_i32, [_i64, C.POINTER(rs_pairs_opts), _pp]),
     "rs_pairs_plan_free": (None, [_p]),
     "rs_pairs_intersect_batched_async": (_i32, [_p, _p, _p, _p, _p, _i64, _p, _p]),

@@ -279,3 +279,53 @@ def test_autotuned_opts_stay_bit_exact(api, oracle_lib):
     ref = oracle_lib.pairs_merge(ha, batch["a"].cpu().numpy(), hb, batch["b"].cpu().numpy())
     assert np.array_equal(ref, sizes.cpu().numpy())
     plan.close()
+
+
+def test_plain_form_reports_overflow(api):
+    """The plain form returns RS_EOVERFLOW where its table cannot hold the
+    larger set (load factor 8, one slot per bucket: rejected before any
+    launch) and where a probe of an absent key walks a completely full table
+    (load factor 2, 16 keys in 16 slots: flagged by the probe kernel)."""
+    from reset_amd.lib import RS_EOVERFLOW, ResetError
+
+    A = np.arange(1000, 1016, dtype=np.int32)          # absent from B
+    B = np.arange(0, 16 * 37, 37, dtype=np.int32)
+    off = np.array([0, 16], np.int64)
+    d = _dev(off, A, off, B)
+    for alpha, b in ((8.0, 1), (2.0, 1)):
+        with pytest.raises(ResetError) as e:
+            api.pairs_intersect_plain(*d, load_factor=alpha, bucket_size=b)
+        assert e.value.status == RS_EOVERFLOW
+    # same pair at the default hyperparameters: exact, and the error word was cleared
+    assert api.pairs_intersect_plain(*d).cpu().tolist() == [0]
+    d2 = _dev(off, B, off, B)
+    assert api.pairs_intersect_plain(*d2, load_factor=2.0, bucket_size=1).cpu().tolist() == [16]
+
+
+def test_batched_flags_unsorted_sets_on_the_chunked_path(api, oracle_lib):
+    """A pair whose smaller set exceeds the bin's chunk is built chunk by
+    chunk in value order: shuffled sets there are reported (RS_EINVAL), not
+    counted.  Pairs whose smaller set fits one table (the resident path) need
+    no order and stay exact when shuffled."""
+    from reset_amd.lib import RS_EINVAL, ResetError
+
+    rng = np.random.default_rng(8)
+    a_off, a, b_off, b, k = G.make_pairs(rng, 200, 3000, 1 << 24)
+    small = (np.diff(a_off) <= 255) | (np.diff(b_off) <= 255)
+    for off, el in ((a_off, a), (b_off, b)):
+        for i in range(len(off) - 1):
+            rng.shuffle(el[off[i]:off[i + 1]])
+    plan = api.PairsPlan(200)
+    with pytest.raises(ResetError) as e:
+        plan.run(*_dev(a_off, a, b_off, b))
+    assert e.value.status == RS_EINVAL
+    # the resident-path pairs alone: exact although shuffled
+    idx = np.flatnonzero(small)
+    assert len(idx) >= 10
+    A = [a[a_off[i]:a_off[i + 1]] for i in idx]
+    B = [b[b_off[i]:b_off[i + 1]] for i in idx]
+    ao = np.concatenate([[0], np.cumsum([len(x) for x in A])]).astype(np.int64)
+    bo = np.concatenate([[0], np.cumsum([len(x) for x in B])]).astype(np.int64)
+    got = plan.run(*_dev(ao, np.concatenate(A), bo, np.concatenate(B))).cpu().numpy()
+    assert np.array_equal(got, k[idx])
+    plan.close()

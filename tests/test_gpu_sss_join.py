@@ -99,3 +99,44 @@ def test_selfjoin_mixes_grouped_and_batched_verification(api, oracle_lib):
     assert np.array_equal(got["pairs"].cpu().numpy(), want)
     big_pairs = [k for k in want.tolist() if (k >> 32) >= nsmall]
     assert len(big_pairs) >= 12 and len(want) > len(big_pairs)
+
+
+@pytest.mark.parametrize("alpha,b", [(0.5, 2), (0.2, 1), (1.0, 4), (0.1, 4)])
+def test_selfjoin_honours_table_hyperparameters(api, oracle_lib, alpha, b):
+    """load_factor / bucket_size set the verification tables (grouped LDS
+    table and the batched kernel's bins for the large build sets); the
+    joined pairs equal the brute force's at every setting."""
+    rng = np.random.default_rng(int(alpha * 100) + b)
+    off, cols = G.zipf_sets(rng, 300, 300, 3, 40)
+    off, cols = _with_near_duplicates(rng, off, cols, 40)
+    rows = [cols[off[i]:off[i + 1]].tolist() for i in range(len(off) - 1)]
+    nsmall = len(rows)
+    for i in range(6):  # build sets above the grouped table at (0.1, 4) and (0.2, 1)
+        big = sorted(set(rng.integers(0, 200000, int(rng.integers(1500, 3000))).tolist()))
+        rows += [big, sorted(set(big[1:] + [400000 + i]))]
+    o = np.zeros(len(rows) + 1, np.int64)
+    np.cumsum([len(r) for r in rows], out=o[1:])
+    off2, cols2 = o.astype(np.int32), np.concatenate([np.array(r, np.int32) for r in rows])
+    want = oracle_lib.sss_bruteforce(off2, cols2, 4, 5)
+    got = api.sss_selfjoin(_cuda(off2), _cuda(cols2), 4, 5, load_factor=alpha, bucket_size=b)
+    assert np.array_equal(got["pairs"].cpu().numpy(), want)
+    assert sum((k >> 32) >= nsmall for k in want.tolist()) >= 6 and len(want) > 12
+
+
+def test_selfjoin_rejects_bad_input_and_settings(api):
+    """A set that is not strictly ascending (out of order, or a repeated
+    token) and verification tables without an LDS form (5-slot buckets, load
+    factor above 1) are rejected with RS_EINVAL, not answered."""
+    from reset_amd.lib import RS_EINVAL, ResetError
+
+    off = np.array([0, 3, 6], np.int32)
+    for cols in ([1, 3, 2, 1, 2, 3], [1, 2, 2, 1, 2, 3], [1, 2, 3, -4, 5, 6]):
+        with pytest.raises(ResetError) as e:
+            api.sss_selfjoin(_cuda(off), _cuda(np.array(cols, np.int32)), 4, 5)
+        assert e.value.status == RS_EINVAL
+    good = _cuda(np.array([1, 2, 3, 1, 2, 3], np.int32))
+    for alpha, b in ((0.2, 5), (1.5, 4), (0.25, 3)):
+        with pytest.raises(ResetError) as e:
+            api.sss_selfjoin(_cuda(off), good, 4, 5, load_factor=alpha, bucket_size=b)
+        assert e.value.status == RS_EINVAL
+    assert api.sss_selfjoin(_cuda(off), good, 4, 5)["pairs"].cpu().tolist() == [1]

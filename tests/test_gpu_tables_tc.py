@@ -235,3 +235,26 @@ def test_tc_hyperparameters_do_not_change_the_count(api, oracle_lib, alpha, b):
     assert t.tc_count(0) == want
     assert t.tc_count(1, probe_keys=t.probe_order(1)) == want
     t.close()
+
+
+@pytest.mark.parametrize("b,lmax", [(9, 2048), (19, 1024)])
+def test_tc_staging_at_the_lds_limit(b, lmax):
+    """Bucket sizes whose largest staged class would fill the 160 KiB LDS
+    exactly (b=9 at L=4096, b=19 at L=2048) without the next-unit word: those
+    tables stay in HBM, the class below is staged, and K_4000 (tables up to
+    L=4096) counts C(n, 3) exactly."""
+    from reset_amd import api as A
+
+    n = 4000
+    deg = np.arange(n - 1, -1, -1, dtype=np.int64)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(deg, out=off[1:])
+    cols = np.concatenate([np.arange(u + 1, n, dtype=np.int32) for u in range(n)])
+    t = A.HashTables(_cuda(off.astype(np.int32)), _cuda(cols), 0.2, b, 3)
+    probes, _ = t.tc_work()
+    staged, units, got_lmax = t.tc_staged()
+    assert got_lmax == lmax and 0 < staged < probes
+    want = math.comb(n, 3)
+    assert t.tc_count(0) == want
+    assert t.tc_count(1, probe_keys=t.probe_order(1)) == want
+    t.close()
