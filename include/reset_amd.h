@@ -93,6 +93,17 @@ typedef struct rs_tables_desc {
 rs_status rs_tables_build(const int32_t* d_offsets, int64_t num_sets, const int32_t* d_elements,
                           float load_factor, int32_t bucket_size, int32_t layouts,
                           rs_stream stream, rs_tables** out);
+/* The build's fill and insert phases alone, over the handle's existing
+ * tables: every slot of the `layouts` tables (a subset of the built ones) is
+ * set to EMPTY, then every (set, element) of the borrowed CSR is inserted
+ * again -- e.g. after the elements were rewritten in place with the same
+ * offsets.  These are the phases the reference times per layout
+ * (triangle-count/hashtable.cu:164-177, build_hash_table_normal /
+ * _hierarchical after their memsets).  No allocation, no synchronization
+ * (enqueue only): the lengths and offsets are kept, the tables must not be
+ * in use by other work on the device, and a full table is reported by
+ * rs_tables_status (RS_EOVERFLOW). */
+rs_status rs_tables_rebuild_async(rs_tables* t, int32_t layouts, rs_stream stream);
 /* Getters of HashTableBuilder (hash_table.cuh:75-80). */
 rs_status rs_tables_describe(const rs_tables* t, rs_tables_desc* out);
 /* Replaces ~HashTableBuilder / free_hash_tables (hash_table.cuh:22-72). */
@@ -100,6 +111,15 @@ void rs_tables_free(rs_tables* t);
 /* Synchronizes `stream` and reports (then clears) device-side error bits set
  * by kernels that used this handle: RS_EOVERFLOW for an exhausted probe. */
 rs_status rs_tables_status(rs_tables* t, rs_stream stream);
+
+/* Kernel timing (no counterpart in the reference, which times with clock()
+ * around a synchronize, TC.cu:299-306): when enabled, rs_tc_count_async and
+ * the IR queries on this handle record HIP events on their stream around
+ * their kernels (TC: the staged and tiled count kernels; IR: the segment
+ * filter and gather); rs_tables_kernel_ms waits for and returns the last
+ * one's duration.  Must be disabled while capturing a graph. */
+rs_status rs_tables_timing(rs_tables* t, int32_t enable);
+rs_status rs_tables_kernel_ms(rs_tables* t, float* h_ms);
 
 /* Row expansion vertexs[e] = row of entry e (triangle-count/hashtable.cu:148-158,
  * TC.cu:277-282, done on the host there; a device kernel here). */
@@ -219,6 +239,13 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
                             int64_t num_queries, const int64_t* d_result_offsets,
                             int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
                             rs_stream stream);
+
+/* Batches of at least `segments` term0 segments (4096 entries each) are
+ * split among waves by work stealing, smaller ones statically; -1 (the
+ * default) means four runs of four segments per wave of the grid.  Results do
+ * not depend on it; it lets a test run either schedule on any input.  Global
+ * to the process; not synchronized with queries in flight. */
+rs_status rs_ir_set_steal_threshold(int64_t segments);
 
 /* Serving loop with continuous batching (no counterpart in the reference,
  * which runs one static batch) for two request kinds: IR conjunctive queries

@@ -118,6 +118,20 @@ class HashTables:
     def status(self, stream=None):
         check(load().rs_tables_status(self._h, stream_handle(stream)))
 
+    def set_timing(self, enable=True):
+        check(load().rs_tables_timing(self._h, int(bool(enable))))
+
+    def kernel_ms(self):
+        """Duration of the last TC count / IR query kernels (HIP events on their stream)."""
+        ms = C.c_float(0)
+        check(load().rs_tables_kernel_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def rebuild_async(self, layouts=BUILD_BOTH, stream=None):
+        """Fill + insert again into the existing tables (no allocation; the
+        build phases the reference times, hashtable.cu:164-177)."""
+        check(load().rs_tables_rebuild_async(self._h, int(layouts), stream_handle(stream)))
+
     def probe_order(self, layout=HIERARCHICAL, stream=None):
         """baseline/common/utils.cu:148-175 (extract_hashtable_to_csr)"""
         out = torch.empty_like(self.elements)
@@ -439,6 +453,11 @@ class PairsPlan:
             self.close()
         except Exception:
             pass
+
+
+def ir_set_steal_threshold(segments=-1):
+    """IR batches of at least `segments` segments use work stealing (-1: default)."""
+    check(load().rs_ir_set_steal_threshold(int(segments)))
 
 
 def pairs_intersect_plain(a_off, a, b_off, b, load_factor=0.2, bucket_size=5, stream=None):

@@ -24,6 +24,7 @@ namespace rs {
 constexpr int kIrBlock = 256;
 constexpr int64_t kIrSeg = 4096;
 constexpr int64_t kIrRun = 4;  // segments per work-stealing take
+constexpr int kIrKeys = 4;     // keys per lane in flight in the filter
 
 // per query: number of segments of term0's list (0 when it has no term or an
 // empty term0 list); validates term ids.
@@ -51,28 +52,103 @@ __global__ void ir_seg_count_kernel(const int32_t* __restrict__ idx_off, int64_t
   if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
 }
 
-template <int LAYOUT>
+// seg_query[g] = the query owning segment g (segments of query q are
+// segoff[q] .. segoff[q+1]).
+__global__ void ir_seg_map_kernel(const int64_t* __restrict__ segoff, int64_t Q,
+                                  int32_t* __restrict__ seg_query) {
+  for (int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < Q;
+       qi += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t g = segoff[qi]; g < segoff[qi + 1]; ++g) seg_query[g] = (int32_t)qi;
+}
+
+// Filters kWave*U keys src[j0 ..] (cur of them valid) through one term's
+// table and appends the survivors, in key order, to dst[w ..].  Each lane
+// holds U keys (key j0 + u*64 + lane in register u) and walks their probe
+// chains together: every step issues the next slot read of all its pending
+// keys before any is compared, so U gathers per lane are in flight (the
+// reference walks one key at a time, ir.cu:72-92).  The walk is TC.cu:11-31's
+// -- slots k = 0..b-1 of the home bucket, then the next bucket mod L --
+// stopping at the key (hit) or an EMPTY slot (miss), bounded at L*b slots.
+template <int LAYOUT, int U>
+__device__ __forceinline__ int32_t ir_filter_round(const int32_t* src, int32_t j0, int32_t cur,
+                                                   const int32_t* __restrict__ Tt, int64_t BN,
+                                                   int32_t b, uint32_t L, uint32_t M, int32_t* dst,
+                                                   int32_t w, bool& bad) {
+  const int lane = __lane_id();
+  int32_t key[U];
+  bool act[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int32_t j = j0 + u * kWave + lane;
+    act[u] = j < cur;
+    key[u] = act[u] ? src[j] : kEmpty;
+  }
+  uint32_t bucket[U];
+  int32_t k[U];
+  bool pend[U], hit[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    bucket[u] = home_bucket<LAYOUT>((uint32_t)key[u], L, M);
+    k[u] = 0;
+    pend[u] = act[u];
+    hit[u] = false;
+  }
+  const uint32_t bound = L * (uint32_t)b;
+  for (uint32_t step = 0; step < bound; ++step) {
+    int32_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = pend[u] ? Tt[bucket[u] + (int64_t)k[u] * BN] : kEmpty;
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!pend[u]) continue;
+      if (v[u] == key[u]) {
+        hit[u] = true;
+        pend[u] = false;
+      } else if (v[u] == kEmpty) {
+        pend[u] = false;
+      } else if (++k[u] == b) {
+        k[u] = 0;
+        bucket[u] = (bucket[u] + 1u) & (L - 1u);
+      }
+      any |= pend[u];
+    }
+    if (!__any(any)) break;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) bad |= pend[u];
+  // stable compaction (ir.cu:151-156): the write position never passes the
+  // keys still to be read (they are all in registers already)
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint64_t m = __ballot(hit[u]);
+    if (hit[u]) dst[w + __popcll(m & lanemask_lt())] = key[u];
+    w += __popcll(m);
+  }
+  return w;
+}
+
+template <int LAYOUT, int U>
 __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
     const int32_t* __restrict__ idx_off, const int32_t* term0_docs,
-    const int32_t* __restrict__ toff32_unused, const int32_t* __restrict__ Ls,
+    const int32_t* __restrict__ seg_query, const int32_t* __restrict__ Ls,
     const int64_t* __restrict__ toff, const int32_t* __restrict__ T, int64_t BN, int32_t b,
-    uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff, int64_t Q,
+    uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff,
     const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t total_segs,
     int32_t* ws, int32_t* __restrict__ seg_count, int32_t* __restrict__ status,
-    unsigned long long* __restrict__ next_run) {
-  (void)toff32_unused;
+    unsigned long long* __restrict__ next_run, int64_t steal_min) {
   const int lane = __lane_id();
   bool bad = false;
-  // A batch of fewer than 4 runs of kIrRun segments per wave (a serving
-  // batch) is dealt statically: wave w takes segments w, w + W, ...  A
-  // larger one uses work stealing (ir.cu:94-176 takes chunks of queries the
-  // same way): a segment's cost varies with its query's term count and
-  // survivors, so waves take runs of kIrRun consecutive segments from a
-  // shared counter, the next run when the current run's last segment starts,
-  // so that its atomic returns meanwhile.
+  // A batch of fewer than steal_min segments (every serving batch) is dealt
+  // statically: wave w takes segments w, w + W, ...  A larger one uses work
+  // stealing (ir.cu:94-176 takes chunks of queries the same way): a
+  // segment's cost varies with its query's term count and survivors, so
+  // waves take runs of kIrRun consecutive segments from a shared counter,
+  // the next run when the current run's last segment starts, so that its
+  // atomic returns meanwhile.
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  const bool steal = total_segs >= 4 * kIrRun * nwaves;
+  const bool steal = total_segs >= steal_min;
   auto take = [&]() -> int64_t {
     unsigned long long v = 0;
     if (lane == 0) v = atomicAdd(next_run, 1ull);
@@ -88,13 +164,7 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
   }
   while (g < total_segs) {
     if (steal && r == kIrRun - 1) q1 = take();
-    // query owning segment g: largest qi with segoff[qi] <= g
-    int64_t lo = 0, hi = Q;
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (segoff[mid] <= g) lo = mid; else hi = mid;
-    }
-    const int64_t qi = lo;
+    const int64_t qi = seg_query[g];
     const int64_t sidx = g - segoff[qi];
     const int32_t qs = qoff[qi], qe = qoff[qi + 1];
     const int32_t t0 = query[qs];
@@ -112,21 +182,8 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
       const int32_t* Tt = T + toff[term];
       const uint32_t L = (uint32_t)Ls[term];
       int32_t w = 0;
-      for (int32_t j0 = 0; j0 < cur; j0 += kWave) {
-        const int32_t j = j0 + lane;
-        int32_t key = kEmpty;
-        bool hit = false;
-        if (j < cur) {
-          key = src[j];
-          const int r = probe(Tt, BN, home_bucket<LAYOUT>((uint32_t)key, L, M), L, b, key);
-          hit = r == 1;
-          bad |= r < 0;
-        }
-        const uint64_t m = __ballot(hit);
-        // ir.cu:151-156: stable in-place compaction (write position <= read)
-        if (hit) dst[w + __popcll(m & lanemask_lt())] = key;
-        w += __popcll(m);
-      }
+      for (int32_t j0 = 0; j0 < cur; j0 += kWave * U)
+        w = ir_filter_round<LAYOUT, U>(src, j0, cur, Tt, BN, b, L, M, dst, w, bad);
       cur = w;
       src = dst;
     }
@@ -168,9 +225,14 @@ __global__ void __launch_bounds__(kIrBlock) ir_gather_kernel(
   }
 }
 
+// Segments below which a batch is dealt statically (-1: 4 runs per wave of
+// the grid); rs_ir_set_steal_threshold changes it, so that tests can run the
+// work-stealing path on small inputs.
+static int64_t ir_steal_min = -1;
+
 // Workspace-carrying enqueue used by the C entry point and by the server.
 struct IrScratch {
-  DevBuf nseg, segoff, seg_count, tmp;
+  DevBuf nseg, segoff, seg_count, seg_query, tmp;
   DevBuf next_run;  // the segment kernel's work-stealing counter
   int64_t total_segs = 0;
 };
@@ -198,24 +260,28 @@ static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* t
   sc.total_segs = total;
   if (sc.seg_count.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
     RS_TRY(sc.seg_count.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+  if (sc.seg_query.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
+    RS_TRY(sc.seg_query.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
+  if (t->timing) RS_HIP(hipEventRecord(t->ev0, st));
   if (total > 0) {
     if (!sc.next_run.p) RS_TRY(sc.next_run.alloc(sizeof(unsigned long long)));
     auto* next_run = sc.next_run.as<unsigned long long>();
     ir_reset_run_kernel<<<1, 1, 0, st>>>(next_run);
+    ir_seg_map_kernel<<<grid_for(Q, 256, 4), 256, 0, st>>>(sc.segoff.as<int64_t>(), Q,
+                                                          sc.seg_query.as<int32_t>());
     const int grid = grid_for(total * kWave, kIrBlock, 8);
-    if (layout == RS_HIERARCHICAL)
-      ir_segment_kernel<RS_HIERARCHICAL><<<grid, kIrBlock, 0, st>>>(
-          t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
-          t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
-          sc.seg_count.as<int32_t>(), t->d_status(), next_run);
-    else
-      ir_segment_kernel<RS_NORMAL><<<grid, kIrBlock, 0, st>>>(
-          t->d_off, term0_docs, nullptr, t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN,
-          t->b, (uint32_t)t->M, query, qoff, Q, roff, sc.segoff.as<int64_t>(), total, ws,
-          sc.seg_count.as<int32_t>(), t->d_status(), next_run);
+    const int64_t nwaves = (int64_t)grid * (kIrBlock / kWave);
+    const int64_t steal_min = ir_steal_min >= 0 ? ir_steal_min : 4 * kIrRun * nwaves;
+    auto kern = layout == RS_HIERARCHICAL ? ir_segment_kernel<RS_HIERARCHICAL, kIrKeys>
+                                          : ir_segment_kernel<RS_NORMAL, kIrKeys>;
+    kern<<<grid, kIrBlock, 0, st>>>(t->d_off, term0_docs, sc.seg_query.as<int32_t>(),
+                                   t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN, t->b,
+                                   (uint32_t)t->M, query, qoff, roff, sc.segoff.as<int64_t>(), total,
+                                   ws, sc.seg_count.as<int32_t>(), t->d_status(), next_run, steal_min);
   }
   ir_gather_kernel<<<grid_for(Q * kWave, kIrBlock, 8), kIrBlock, 0, st>>>(
       roff, sc.segoff.as<int64_t>(), Q, ws, sc.seg_count.as<int32_t>(), result, count);
+  if (t->timing) RS_HIP(hipEventRecord(t->ev1, st));
   RS_HIP(hipGetLastError());
   return RS_OK;
 }
@@ -634,6 +700,12 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
                     num_queries, d_result_offsets, d_result, d_result_count, d_workspace, sc, st,
                     true, 0));
   RS_HIP(hipStreamSynchronize(st));  // scratch is released on return
+  return RS_OK;
+}
+
+rs_status rs_ir_set_steal_threshold(int64_t segments) {
+  RS_REQUIRE(segments >= -1, "rs_ir_set_steal_threshold: %lld is below -1", (long long)segments);
+  ir_steal_min = segments;
   return RS_OK;
 }
 

@@ -108,6 +108,44 @@ __global__ void __launch_bounds__(256) probe_order_kernel(const int32_t* __restr
   }
 }
 
+// Fills n4 16-byte groups with EMPTY (a kernel rather than a memset node, so
+// the sequence stays capturable).
+__global__ void fill_empty_kernel(int4* __restrict__ p, int64_t n4) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = int4{kEmpty, kEmpty, kEmpty, kEmpty};
+}
+
+// a4 + a6 for the layouts in `mask` (tables allocated): EMPTY fill, then the
+// insert of every (row, element).  Conflicts are added to the counters at
+// status+16 (normal) and status+24 (hierarchical).
+static rs_status fill_and_insert(rs_tables* t, int32_t mask, hipStream_t st) {
+  // BN is a multiple of 8 (every length is a power of two >= 8), so a plane
+  // holds whole 16-byte groups
+  const int64_t n4 = t->BN * (int64_t)t->b / 4;
+  const int fg = grid_for(n4, 256, 16);
+  if (mask & RS_BUILD_NORMAL) fill_empty_kernel<<<fg, 256, 0, st>>>(t->t_normal.as<int4>(), n4);
+  if (mask & RS_BUILD_HIERARCHICAL) fill_empty_kernel<<<fg, 256, 0, st>>>(t->t_hier.as<int4>(), n4);
+  if (t->E > 0) {
+    auto* conf = reinterpret_cast<unsigned long long*>(t->status.as<char>() + 16);
+    const int grid = grid_for(t->E, 256, 16);
+    if (mask == RS_BUILD_BOTH)
+      build_kernel<true, true><<<grid, 256, 0, st>>>(
+          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
+          t->BN, t->b, t->M, t->t_normal.as<int32_t>(), t->t_hier.as<int32_t>(), t->d_status(), conf);
+    else if (mask == RS_BUILD_NORMAL)
+      build_kernel<true, false><<<grid, 256, 0, st>>>(
+          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
+          t->BN, t->b, t->M, t->t_normal.as<int32_t>(), nullptr, t->d_status(), conf);
+    else
+      build_kernel<false, true><<<grid, 256, 0, st>>>(
+          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
+          t->BN, t->b, t->M, nullptr, t->t_hier.as<int32_t>(), t->d_status(), conf);
+  }
+  RS_HIP(hipGetLastError());
+  return RS_OK;
+}
+
 }  // namespace rs
 
 using namespace rs;
@@ -174,35 +212,11 @@ rs_status rs_tables_build(const int32_t* d_offsets, int64_t num_sets, const int3
   if ((s = t->rows.alloc(sizeof(int32_t) * (size_t)t->E)) != RS_OK) return fail(s);
   if ((s = expand_rows(d_offsets, num_sets, t->E, t->rows.as<int32_t>(), st)) != RS_OK)
     return fail(s);
-  // a4: tables filled with EMPTY
+  // a4 + a6: tables filled with EMPTY, then the insert
   const size_t tbytes = sizeof(int32_t) * (size_t)t->BN * (size_t)bucket_size;
-  if (layouts & RS_BUILD_NORMAL) {
-    if ((s = t->t_normal.alloc(tbytes)) != RS_OK) return fail(s);
-    if (hipMemsetAsync(t->t_normal.p, 0xFF, tbytes, st) != hipSuccess) return fail(RS_EDEVICE);
-  }
-  if (layouts & RS_BUILD_HIERARCHICAL) {
-    if ((s = t->t_hier.alloc(tbytes)) != RS_OK) return fail(s);
-    if (hipMemsetAsync(t->t_hier.p, 0xFF, tbytes, st) != hipSuccess) return fail(RS_EDEVICE);
-  }
-  // a6: insert
-  if (t->E > 0) {
-    auto* conf = reinterpret_cast<unsigned long long*>(t->status.as<char>() + 16);
-    const int grid = grid_for(t->E, 256, 16);
-    if (layouts == RS_BUILD_BOTH)
-      build_kernel<true, true><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), d_elements, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, bucket_size, t->M, t->t_normal.as<int32_t>(), t->t_hier.as<int32_t>(),
-          t->d_status(), conf);
-    else if (layouts == RS_BUILD_NORMAL)
-      build_kernel<true, false><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), d_elements, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, bucket_size, t->M, t->t_normal.as<int32_t>(), nullptr, t->d_status(), conf);
-    else
-      build_kernel<false, true><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), d_elements, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, bucket_size, t->M, nullptr, t->t_hier.as<int32_t>(), t->d_status(), conf);
-    if (hipGetLastError() != hipSuccess) return fail(RS_EDEVICE);
-  }
+  if ((layouts & RS_BUILD_NORMAL) && (s = t->t_normal.alloc(tbytes)) != RS_OK) return fail(s);
+  if ((layouts & RS_BUILD_HIERARCHICAL) && (s = t->t_hier.alloc(tbytes)) != RS_OK) return fail(s);
+  if ((s = fill_and_insert(t, layouts, st)) != RS_OK) return fail(s);
   int32_t err = 0;
   unsigned long long conf[2] = {0, 0};
   if (hipMemcpyAsync(&err, t->status.p, sizeof(err), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -222,6 +236,31 @@ rs_status rs_tables_build(const int32_t* d_offsets, int64_t num_sets, const int3
     return fail(RS_EOVERFLOW);
   }
   *out = t;
+  return RS_OK;
+}
+
+rs_status rs_tables_rebuild_async(rs_tables* t, int32_t layouts, rs_stream stream) {
+  RS_REQUIRE(t, "rs_tables_rebuild_async: null handle");
+  RS_REQUIRE(layouts >= 1 && layouts <= 3 && (layouts & t->layouts) == layouts,
+             "rs_tables_rebuild_async: layouts mask %d names a layout that was not built (built: %d)",
+             layouts, t->layouts);
+  return fill_and_insert(t, layouts, as_stream(stream));
+}
+
+rs_status rs_tables_timing(rs_tables* t, int32_t enable) {
+  RS_REQUIRE(t, "rs_tables_timing: null handle");
+  if (enable && !t->ev0) {
+    RS_HIP(hipEventCreate(&t->ev0));
+    RS_HIP(hipEventCreate(&t->ev1));
+  }
+  t->timing = enable != 0;
+  return RS_OK;
+}
+
+rs_status rs_tables_kernel_ms(rs_tables* t, float* h_ms) {
+  RS_REQUIRE(t && h_ms && t->ev0, "rs_tables_kernel_ms: timing was never enabled");
+  RS_HIP(hipEventSynchronize(t->ev1));
+  RS_HIP(hipEventElapsedTime(h_ms, t->ev0, t->ev1));
   return RS_OK;
 }
 

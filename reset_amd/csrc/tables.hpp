@@ -20,6 +20,14 @@ struct rs_tables {
   rs::DevBuf status;     // int32 error bits, then 2 x uint64 conflict counters
   int64_t conflicts[2] = {0, 0};
   rs_tc_plan* tc_plan = nullptr;  // cached by rs_tc_count
+  // rs_tables_timing: events around the kernels of the last TC count / IR
+  // query on this handle
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~rs_tables() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
 
   const int32_t* table(int32_t layout) const {
     return layout == RS_HIERARCHICAL ? t_hier.as<int32_t>() : t_normal.as<int32_t>();

@@ -965,10 +965,12 @@ rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t lay
   auto* out = reinterpret_cast<unsigned long long*>(d_count);
   zero_u64<<<1, 1, 0, st>>>(out);
   const int32_t* keys = d_probe_keys ? d_probe_keys : t->d_elems;
+  if (t->timing) RS_HIP(hipEventRecord(t->ev0, st));
   if (layout == RS_HIERARCHICAL)
     RS_TRY(tc_count_layout<RS_HIERARCHICAL>(t, p, T, keys, out, st));
   else
     RS_TRY(tc_count_layout<RS_NORMAL>(t, p, T, keys, out, st));
+  if (t->timing) RS_HIP(hipEventRecord(t->ev1, st));
   RS_HIP(hipGetLastError());
   return RS_OK;
 }

@@ -182,3 +182,32 @@ def ir_queries(index_offsets, num_queries, kmin=2, kmax=8, seed=20261203):
         terms.extend(got)
         qoff.append(len(terms))
     return np.array(terms, np.int32), np.array(qoff, np.int32)
+
+
+def plant_near_duplicates(off, cols, count, seed=20261213, vocab=1_000_000):
+    """Replace the last `count` sets of a collection by copies of distinct
+    earlier sets with one token swapped for a vocabulary token the set lacks:
+    J(copy, source) = (s - 1) / (s + 1) >= 4/5 for sets of s >= 9 tokens.
+    -> (offsets, elements, planted) with planted the int64 keys
+    source << 32 | copy (source < copy), ascending."""
+    rng = np.random.default_rng(seed)
+    offh = off.cpu().numpy().astype(np.int64)
+    n = len(offh) - 1
+    count = min(count, n // 2)
+    keep = int(offh[n - count])
+    tail, planted = [], []
+    for j, src in enumerate(rng.choice(n - count, count, replace=False).tolist()):
+        r = cols[int(offh[src]):int(offh[src + 1])].cpu().numpy().copy()
+        have = set(r.tolist())
+        x = int(rng.integers(0, vocab))
+        while x in have:
+            x = int(rng.integers(0, vocab))
+        r[int(rng.integers(0, len(r)))] = x
+        tail.append(np.sort(r))
+        planted.append((src << 32) | (n - count + j))
+    lens = np.array([len(r) for r in tail], np.int64)
+    new_off = offh.copy()
+    new_off[n - count + 1:] = keep + np.cumsum(lens)
+    new_cols = torch.cat([cols[:keep], torch.from_numpy(np.concatenate(tail).astype(np.int32)).to(cols.device)])
+    return (torch.from_numpy(new_off.astype(np.int32)).to(off.device), new_cols,
+            np.sort(np.array(planted, np.int64)))

@@ -64,6 +64,10 @@ SIGNATURES = {
     "rs_calculate_length": (_i32, [_i32, _f32, _i32]),
     "rs_calculate_lengths": (_i32, [_p, _i64, _f32, _i32, _p, _p, _p]),
     "rs_tables_build": (_i32, [_p, _i64, _p, _f32, _i32, _i32, _p, _pp]),
+    "rs_tables_rebuild_async": (_i32, [_p, _i32, _p]),
+    "rs_tables_timing": (_i32, [_p, _i32]),
+    "rs_tables_kernel_ms": (_i32, [_p, C.POINTER(C.c_float)]),
+    "rs_ir_set_steal_threshold": (_i32, [_i64]),
     "rs_tables_describe": (_i32, [_p, C.POINTER(rs_tables_desc)]),
     "rs_tables_free": (None, [_p]),
     "rs_tables_status": (_i32, [_p, _p]),

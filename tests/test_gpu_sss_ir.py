@@ -145,6 +145,27 @@ def test_ir_long_term0_lists_multi_segment(api, oracle_lib):
     t.close()
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("steal_min", [0, 1, 7])
+def test_ir_work_stealing_schedule_matches_merge(api, oracle_lib, layout, steal_min):
+    """The work-stealing segment schedule (forced on small batches through
+    the steal threshold) returns the chained merge's lists, ascending, for
+    both layouts with term0 read from the index."""
+    rng = np.random.default_rng(30 + steal_min)
+    idx_off, docs, q, qoff = _ir_case(rng, nterms=200, ndocs=400000, maxlen=40000, nq=300, kmax=8)
+    t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 3)
+    m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
+    try:
+        api.ir_set_steal_threshold(steal_min)
+        res, cnt, roff = t.ir_query(_cuda(q), _cuda(qoff), layout)
+    finally:
+        api.ir_set_steal_threshold(-1)
+    assert np.array_equal(cnt.cpu().numpy(), m_cnt)
+    assert oracle_lib.ir_result_lists(res.cpu().numpy(), cnt.cpu().numpy(), m_roff) == \
+        oracle_lib.ir_result_lists(m_res, m_cnt, m_roff)
+    t.close()
+
+
 def test_ir_special_queries(api, oracle_lib):
     idx_off = np.array([0, 3, 3, 6], np.int32)
     docs = np.array([1, 5, 9, 5, 9, 12], np.int32)

@@ -75,6 +75,35 @@ def test_table_geometry_and_invariants(api, oracle_lib, layout_mask):
     t.close()
 
 
+def test_rebuild_in_place_keeps_invariants(api, oracle_lib):
+    """rs_tables_rebuild_async (fill + insert, no allocation) over the same
+    CSR, then over rewritten elements with the same offsets: the tables meet
+    the oracle's invariants for the elements they now hold, and a layout
+    that was not built is refused."""
+    from reset_amd.lib import RS_EINVAL, ResetError
+
+    rng = np.random.default_rng(21)
+    off, cols = G.random_sets(rng, 2000, 500, 1 << 24)
+    cd = _cuda(cols)
+    t = api.HashTables(_cuda(off), cd, 0.2, 5, 3)
+    L, toff, M = oracle_lib.geometry(off, 0.2, 5)
+    for new in (cols, (cols.astype(np.int64) * 7 + 3) % (1 << 24)):
+        cd.copy_(_cuda(new))
+        t.rebuild_async(3)
+        t.rebuild_async(2)
+        torch.cuda.synchronize()
+        t.status()
+        for layout in (0, 1):
+            ref = oracle_lib.Tables(L, toff, M, 5, layout, None, 0)
+            assert oracle_lib.table_check(off, new.astype(np.int32), ref, T=t.table(layout).cpu().numpy()) == 0
+    t.close()
+    t1 = api.HashTables(_cuda(off), cd, 0.2, 5, 1)
+    with pytest.raises(ResetError) as e:
+        t1.rebuild_async(2)
+    assert e.value.status == RS_EINVAL
+    t1.close()
+
+
 def test_table_overflow_is_reported(api):
     off = np.array([0, 200], np.int32)
     cols = np.arange(200, dtype=np.int32)
