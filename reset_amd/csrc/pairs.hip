@@ -393,7 +393,7 @@ template <int B, int U>
 __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict__ S, int32_t ns,
                                                       const int32_t* __restrict__ G, int32_t ng,
                                                       int32_t* tab, int32_t* ring, float alpha,
-                                                      int32_t chunk, bool& bad, bool& unordered) {
+                                                      int32_t chunk, bool& bad, uint32_t& unordered) {
   constexpr int32_t kBlock = 4 * U * kWave;
   const int lane = __lane_id();
   // the bin's table: calculate_length(chunk) buckets (fits the wave's region)
@@ -450,7 +450,7 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       spos[r] = -1;
       if (idx < ce) {
         const int32_t v = ring_at(idx);
-        unordered |= (uint32_t)v < lo || (uint32_t)v > hi;
+        unordered |= (uint32_t)((uint32_t)v < lo) | (uint32_t)((uint32_t)v > hi);
         spos[r] = lds_insert<B>(tab, L, v, moved);
         bad |= spos[r] < 0;
       }
@@ -470,6 +470,11 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
     load_block<U>(G, pos + kBlock, 0, ng, nxt);  // unconditional, as above
     const int32_t staged_g = sc;
     const v4i staged = group_load(staged_g);     // unconditional, as above
+    // an element below lo can only be met on the block's first pass: the
+    // elements left for a later chunk lie above this chunk's top, which is
+    // one below the next chunk's lo
+#pragma unroll
+    for (int i = 0; i < 4 * U; ++i) unordered |= (uint32_t)cur.act[i] & (uint32_t)((uint32_t)cur.x[i] < lo);
     while (true) {
       bool in[4 * U];
       bool rest = false;
@@ -477,7 +482,6 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       for (int i = 0; i < 4 * U; ++i) {
         const bool below = (uint32_t)cur.x[i] <= top;
         in[i] = cur.act[i] && below;
-        unordered |= in[i] && (uint32_t)cur.x[i] < lo;
         cur.act[i] = cur.act[i] && !below;
         rest |= cur.act[i];
       }
@@ -573,7 +577,10 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     idx = (int64_t)blockIdx.x * kWavesPerBlock + wid;
   }
   int64_t nidx = following(idx);
-  bool bad = false, unordered = false;
+  bool bad = false;
+  // out-of-order elements met on the chunked path (a per-lane word, not a
+  // lane mask: it stays out of the scalar registers the loops need)
+  uint32_t unordered = 0;
   int32_t i = 0;
   const int32_t* A = nullptr;
   const int32_t* Bp = nullptr;
@@ -632,7 +639,7 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     nidx = nnidx;
   }
   if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
-  if (__any(unordered) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrBadInput);
+  if (__any(unordered != 0) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrBadInput);
 }
 
 // ---------------------------------------------------------------- plain form
