@@ -512,6 +512,44 @@ def _ir_probe_work(t, q, qoff, layout):
     return total
 
 
+def bench_ir_batch(args):
+    """All IR queries of the 10M-document index in one batch, hierarchical
+    tables, K times: the roofline of the query kernels alone (the command the
+    IR counter profiles run)."""
+    import numpy as np
+    import torch
+
+    from reset_amd import api, inputs
+
+    off, docs = inputs.ir_index(seed=SEED + 5)
+    t = api.HashTables(off, docs, 0.2, 5, 2)
+    q, qoff = inputs.ir_queries(off, args.queries, seed=SEED + 6)
+    qd, qod = torch.from_numpy(q).cuda(), torch.from_numpy(qoff).cuda()
+    for _ in range(args.warmup):
+        t.ir_query(qd, qod, 1)
+    t.set_timing(True)
+    kms = []
+    for _ in range(args.steps):
+        _, cnt, roff = t.ir_query(qd, qod, 1)
+        kms.append(t.kernel_ms())
+    t.set_timing(False)
+    probes = _ir_probe_work(t, q, qoff, 1)
+    n0 = int(roff[-1].item())
+    alg = 8 * n0 + 8 * probes
+    ms = float(np.mean(kms))
+    t.close()
+    return {"metric": "IR batch of all queries: query kernels (1 GPU)", "value": ms, "unit": "ms",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic Zipf(1.1) inverted index (seeded on device)",
+            "config": {"workload": f"{len(qoff) - 1} conjunctive queries of 2-8 terms over 10M docs / 1M terms, "
+                                   "one batch, hierarchical tables", "layout": "hierarchical"},
+            "results": int(cnt.sum().item()),
+            "roofline": {"bound": "hbm", "achieved": alg / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "kernel_ms": ms,
+                         "alg_bytes_per_launch": alg, "term0_entries": n0, "probes": probes}}
+
+
 def _serve_at(t, queries, qps, n, args, seed, layout):
     """Poisson arrivals at `qps` of the first n queries through the serving
     loop -> (latencies ms, span ms, results by id, stats)."""
@@ -931,7 +969,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "pairs", "tc", "sss", "ir", "compare", "pairs_shapes"])
+                    choices=["all", "pairs", "tc", "sss", "ir", "ir_batch", "compare", "pairs_shapes"])
     ap.add_argument("--sets", type=int, default=1_000_000)
     ap.add_argument("--queries", type=int, default=20_000)
     ap.add_argument("--qps", type=float, default=2000.0, help="ir: the QPS whose latency heads the line")
@@ -985,6 +1023,7 @@ def main(argv=None):
     if args.scale is None:
         args.scale = 20 if args.workload == "compare" else 24
     out = {"all": bench_all, "pairs": bench_pairs, "tc": bench_tc, "sss": bench_sss, "ir": bench_ir,
+           "ir_batch": bench_ir_batch,
            "compare": bench_compare, "pairs_shapes": bench_pairs_shapes}[args.workload](args)
     print(json.dumps(out), flush=True)
     return 0

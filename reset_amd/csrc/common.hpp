@@ -175,6 +175,11 @@ rs_status exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, 
                                     hipStream_t stream);
 rs_status exclusive_scan_i64(const int64_t* in, int64_t n, int64_t* out, DevBuf& tmp,
                              hipStream_t stream);
+// The same with caller-provided tile sums (scan_tile_count(n) int64 of them):
+// no allocation.
+int64_t scan_tile_count(int64_t n);
+rs_status exclusive_scan_i32_to_i64_buf(const int32_t* in, int64_t n, int64_t* out, int64_t* sums,
+                                        int64_t sums_cap, hipStream_t stream);
 
 // rows[e] = the CSR row that holds entry e.
 rs_status expand_rows(const int32_t* off, int64_t n, int64_t E, int32_t* rows, hipStream_t stream);

@@ -114,22 +114,43 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles(const Tin* __restrict__
   }
 }
 
+template <typename T>
+__global__ void scan_zero_total(T* p) { *p = T(0); }
+
+// With caller-provided tile sums (at least scan_tile_count(n) of them).
 template <typename Tin, typename Tout, bool INCLUSIVE>
-static rs_status scan_impl(const Tin* in, int64_t n, Tout* out, Tout* total_out, DevBuf& tmp,
-                           hipStream_t stream) {
+static rs_status scan_impl_buf(const Tin* in, int64_t n, Tout* out, Tout* total_out, Tout* sums,
+                               int64_t sums_cap, hipStream_t stream) {
   if (n <= 0) {
-    if (total_out) RS_HIP(hipMemsetAsync(total_out, 0, sizeof(Tout), stream));
+    if (total_out) scan_zero_total<Tout><<<1, 1, 0, stream>>>(total_out);
+    RS_HIP(hipGetLastError());
     return RS_OK;
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   RS_REQUIRE(tiles < (1ll << 31), "scan of %lld items is too large", (long long)n);
-  if (tmp.bytes < sizeof(Tout) * (size_t)tiles) RS_TRY(tmp.alloc(sizeof(Tout) * (size_t)tiles));
-  Tout* sums = tmp.as<Tout>();
+  RS_REQUIRE(sums && tiles <= sums_cap, "scan of %lld items needs %lld tile sums (have %lld)",
+             (long long)n, (long long)tiles, (long long)sums_cap);
   scan_tile_sums<Tin, Tout><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, n, sums);
   scan_sums_single<Tout><<<1, kScanBlock, 0, stream>>>(sums, tiles, total_out);
   scan_tiles<Tin, Tout, INCLUSIVE><<<(unsigned)tiles, kScanBlock, 0, stream>>>(in, n, sums, out);
   RS_HIP(hipGetLastError());
   return RS_OK;
+}
+
+template <typename Tin, typename Tout, bool INCLUSIVE>
+static rs_status scan_impl(const Tin* in, int64_t n, Tout* out, Tout* total_out, DevBuf& tmp,
+                           hipStream_t stream) {
+  const int64_t tiles = n > 0 ? (n + kScanTile - 1) / kScanTile : 0;
+  if (tiles > 0 && tmp.bytes < sizeof(Tout) * (size_t)tiles) RS_TRY(tmp.alloc(sizeof(Tout) * (size_t)tiles));
+  return scan_impl_buf<Tin, Tout, INCLUSIVE>(in, n, out, total_out, tmp.as<Tout>(),
+                                             (int64_t)(tmp.bytes / sizeof(Tout)), stream);
+}
+
+int64_t scan_tile_count(int64_t n) { return n > 0 ? (n + kScanTile - 1) / kScanTile : 0; }
+
+rs_status exclusive_scan_i32_to_i64_buf(const int32_t* in, int64_t n, int64_t* out, int64_t* sums,
+                                        int64_t sums_cap, hipStream_t stream) {
+  return scan_impl_buf<int32_t, int64_t, false>(in, n, out, out + n, sums, sums_cap, stream);
 }
 
 rs_status exclusive_scan_i32_to_i64(const int32_t* in, int64_t n, int64_t* out, DevBuf& tmp,

@@ -1,18 +1,21 @@
 // ir.hip -- conjunctive queries over an inverted index (information-retrieval/
 // ir.cu:94-176) and a continuous-batching serving loop around them.
 //
-// Query kernel: term0's list (the shortest: queries are ordered by document
+// Query kernels: term0's list (the shortest: queries are ordered by document
 // frequency) is cut into segments of kIrSeg entries; each wave filters one
-// segment through every later term's table, compacting survivors in place
-// with a 64-bit ballot (stable, so survivors keep term0's order).  Because
-// intersection distributes over the segments, a 2M-entry term0 list is spread
-// over hundreds of waves instead of one.  A second pass concatenates each
-// query's segment survivors into its result region (ir.cu:324-330 sizing).
+// segment through every later term's table, four keys per lane in flight,
+// compacting survivors in place with a 64-bit ballot (stable, so survivors
+// keep term0's order).  Because intersection distributes over the segments,
+// a 2M-entry term0 list is spread over hundreds of waves instead of one.  A
+// prefix scan of the segments' survivor counts then places every segment's
+// survivors at once: in each query's result region (ir.cu:324-330 sizing) or
+// densely, query after query, for the server.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -25,6 +28,7 @@ constexpr int kIrBlock = 256;
 constexpr int64_t kIrSeg = 4096;
 constexpr int64_t kIrRun = 4;  // segments per work-stealing take
 constexpr int kIrKeys = 4;     // keys per lane in flight in the filter
+constexpr int kIrXcds = 8;     // XCDs of the MI355X (per-XCD work ranges)
 
 // per query: number of segments of term0's list (0 when it has no term or an
 // empty term0 list); validates term ids.
@@ -131,39 +135,42 @@ __device__ __forceinline__ int32_t ir_filter_round(const int32_t* src, int32_t j
 template <int LAYOUT, int U>
 __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
     const int32_t* __restrict__ idx_off, const int32_t* term0_docs,
-    const int32_t* __restrict__ seg_query, const int32_t* __restrict__ Ls,
-    const int64_t* __restrict__ toff, const int32_t* __restrict__ T, int64_t BN, int32_t b,
-    uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff,
+    const int32_t* __restrict__ seg_query, const int32_t* __restrict__ Ls, const int64_t* __restrict__ toff, const int32_t* __restrict__ T,
+    int64_t BN, int32_t b, uint32_t M, const int32_t* __restrict__ query, const int32_t* __restrict__ qoff,
     const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t total_segs,
     int32_t* ws, int32_t* __restrict__ seg_count, int32_t* __restrict__ status,
-    unsigned long long* __restrict__ next_run, int64_t steal_min) {
+    unsigned long long* __restrict__ next_run, bool steal) {
   const int lane = __lane_id();
   bool bad = false;
-  // A batch of fewer than steal_min segments (every serving batch) is dealt
-  // statically: wave w takes segments w, w + W, ...  A larger one uses work
-  // stealing (ir.cu:94-176 takes chunks of queries the same way): a
-  // segment's cost varies with its query's term count and survivors, so
-  // waves take runs of kIrRun consecutive segments from a shared counter,
-  // the next run when the current run's last segment starts, so that its
-  // atomic returns meanwhile.
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  // A small batch (every serving batch) is dealt statically: wave w takes
+  // segments w, w + W, ...  A large one is cut into kIrXcds contiguous ranges
+  // of segments, one per XCD (blocks are dealt to the XCDs round robin), whose
+  // waves take runs of kIrRun segments from their XCD's counter, the next run
+  // when the current run's last segment starts, so that its atomic returns
+  // meanwhile (ir.cu:94-176 takes chunks of queries from one counter): a
+  // segment's cost varies with its query's term count and survivors, and a
+  // query's segments, which probe the same tables, stay on one XCD.
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  const bool steal = total_segs >= steal_min;
+  const int xcd = (int)(blockIdx.x % kIrXcds);
+  const int64_t xlo = steal ? total_segs * xcd / kIrXcds : 0;
+  const int64_t xhi = steal ? total_segs * (xcd + 1) / kIrXcds : total_segs;
+  unsigned long long* const ctr = next_run + xcd;
   auto take = [&]() -> int64_t {
     unsigned long long v = 0;
-    if (lane == 0) v = atomicAdd(next_run, 1ull);
+    if (lane == 0) v = atomicAdd(ctr, 1ull);
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo) * kIrRun;
+    return xlo + (int64_t)(((uint64_t)hi << 32) | lo) * kIrRun;
   };
-  int64_t q0 = 0, q1 = 0, g = wave;
+  int64_t q0 = 0, q1 = 0, p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   int32_t r = 0;
   if (steal) {
     q0 = take();
-    g = q0;
+    p = q0;
   }
-  while (g < total_segs) {
+  while (p < xhi) {
     if (steal && r == kIrRun - 1) q1 = take();
+    const int64_t g = p;
     const int64_t qi = seg_query[g];
     const int64_t sidx = g - segoff[qi];
     const int32_t qs = qoff[qi], qe = qoff[qi + 1];
@@ -189,39 +196,62 @@ __global__ void __launch_bounds__(kIrBlock) ir_segment_kernel(
     }
     if (lane == 0) seg_count[g] = cur;
     if (!steal) {
-      g += nwaves;
+      p += nwaves;
     } else if (++r < kIrRun) {
-      g = q0 + r;
+      p = q0 + r;
     } else {
       r = 0;
       q0 = q1;
-      g = q0;
+      p = q0;
     }
+    // (a run that reaches past xhi ends the loop there: the positions
+    // beyond belong to the next XCD's range, and every later run of this
+    // counter starts beyond them too)
   }
   if (__any(bad) && lane == 0) atomicOr(status, kErrProbeBound);
 }
 
-__global__ void ir_reset_run_kernel(unsigned long long* p) { *p = 0ull; }
+__global__ void ir_reset_runs_kernel(unsigned long long* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
 
-// One wave per query: concatenate the survivors of its segments into
-// result[roff[q] ..] and write count[q].
-__global__ void __launch_bounds__(kIrBlock) ir_gather_kernel(
-    const int64_t* __restrict__ roff, const int64_t* __restrict__ segoff, int64_t Q,
-    const int32_t* __restrict__ ws, const int32_t* __restrict__ seg_count,
-    int32_t* __restrict__ result, int32_t* __restrict__ count) {
+// count[q] (and, for the dense form, out_off[q]) from seg_pos, the exclusive
+// prefix of the segments' survivor counts in segment order.
+__global__ void ir_counts_kernel(const int64_t* __restrict__ segoff, int64_t Q,
+                                 const int64_t* __restrict__ seg_pos, int32_t* __restrict__ count,
+                                 int64_t* __restrict__ out_off) {
+  for (int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < Q;
+       qi += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = seg_pos[segoff[qi]], e = seg_pos[segoff[qi + 1]];
+    count[qi] = (int32_t)(e - a);
+    if (out_off) {
+      out_off[qi] = a;
+      if (qi == Q - 1) out_off[Q] = e;
+    }
+  }
+}
+
+// One wave per segment: its survivors (a stable run in term0 order) go to
+// the query's result region after those of the query's earlier segments --
+// result[roff[q] + seg_pos[g] - seg_pos[segoff[q]] ..], or to
+// dense[seg_pos[g] ..] for the dense form (every query's result back to
+// back, in query order).
+__global__ void __launch_bounds__(kIrBlock) ir_copy_kernel(
+    const int32_t* __restrict__ seg_query, const int64_t* __restrict__ segoff,
+    const int64_t* __restrict__ roff, int64_t total, const int32_t* __restrict__ ws,
+    const int32_t* __restrict__ seg_count, const int64_t* __restrict__ seg_pos,
+    int32_t* __restrict__ result, int32_t* __restrict__ dense) {
   const int lane = __lane_id();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  for (int64_t qi = wave; qi < Q; qi += nwaves) {
-    const int64_t g0 = segoff[qi], g1 = segoff[qi + 1];
-    int64_t w = roff[qi];
-    for (int64_t g = g0; g < g1; ++g) {
-      const int32_t c = seg_count[g];
-      const int32_t* src = ws + roff[qi] + (g - g0) * kIrSeg;
-      for (int32_t j = lane; j < c; j += kWave) result[w + j] = src[j];
-      w += c;
-    }
-    if (lane == 0) count[qi] = (int32_t)(w - roff[qi]);
+  for (int64_t g = wave; g < total; g += nwaves) {
+    const int32_t c = seg_count[g];
+    if (c == 0) continue;
+    const int64_t qi = seg_query[g];
+    const int64_t g0 = segoff[qi];
+    const int32_t* src = ws + roff[qi] + (g - g0) * kIrSeg;
+    int32_t* dst = dense ? dense + seg_pos[g] : result + roff[qi] + (seg_pos[g] - seg_pos[g0]);
+    for (int32_t j = lane; j < c; j += kWave) dst[j] = src[j];
   }
 }
 
@@ -230,98 +260,125 @@ __global__ void __launch_bounds__(kIrBlock) ir_gather_kernel(
 // work-stealing path on small inputs.
 static int64_t ir_steal_min = -1;
 
-// Workspace-carrying enqueue used by the C entry point and by the server.
+// Device scratch of one query batch, as plain pointers: carved from DevBufs
+// (IrScratch, the allocating entry points and the server) or from a fixed
+// region (the allocation-free entry point).
+struct IrWork {
+  int32_t* nseg = nullptr;       // [Q]
+  int64_t* segoff = nullptr;     // [Q+1]
+  int64_t* sums = nullptr;       // scan tile sums
+  int64_t sums_cap = 0;
+  int32_t* seg_query = nullptr;  // [S]
+  int32_t* seg_count = nullptr;  // [S]
+  int64_t* seg_pos = nullptr;    // [S+1]
+  unsigned long long* next_run = nullptr;  // [kIrXcds] per-XCD work-stealing counters
+};
+
+// The allocating form: buffers grown as batches need them.
 struct IrScratch {
-  DevBuf nseg, segoff, seg_count, seg_query, tmp;
-  DevBuf next_run;  // the segment kernel's work-stealing counter
+  DevBuf nseg, segoff, sums, seg_query, seg_count, seg_pos, next_run;
   int64_t total_segs = 0;
 };
 
-static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* term0_docs,
-                            const int32_t* query, const int32_t* qoff, int64_t Q,
-                            const int64_t* roff, int32_t* result, int32_t* count, int32_t* ws,
-                            IrScratch& sc, hipStream_t st, bool sync_total, int64_t known_segs) {
+template <typename T>
+static rs_status grow(DevBuf& b, int64_t n) {
+  const size_t need = sizeof(T) * (size_t)(n > 0 ? n : 1);
+  if (b.bytes < need) RS_TRY(b.alloc(need));
+  return RS_OK;
+}
+
+// Phase 1: segments per query and their offsets (segoff[Q] = total).
+static rs_status ir_plan_segments(const rs_tables* t, const int32_t* query, const int32_t* qoff, int64_t Q,
+                                  const int64_t* roff, const IrWork& w, hipStream_t st) {
+  ir_seg_count_kernel<<<grid_for(Q, 256, 4), 256, 0, st>>>(t->d_off, t->n, query, qoff, Q, roff, w.nseg,
+                                                          t->d_status());
+  RS_TRY(exclusive_scan_i32_to_i64_buf(w.nseg, Q, w.segoff, w.sums, w.sums_cap, st));
+  return RS_OK;
+}
+
+// Phase 2 (total segments known on the host): the segment filter, then the
+// survivors' positions and the copy into result (roff layout) or dense.
+static rs_status ir_run_segments(const rs_tables* t, int32_t layout, const int32_t* term0_docs,
+                                 const int32_t* query, const int32_t* qoff, int64_t Q, const int64_t* roff,
+                                 int32_t* result, int32_t* count, int32_t* ws, int32_t* dense,
+                                 int64_t* out_off, int64_t total, const IrWork& w, hipStream_t st) {
   const int32_t* T = nullptr;
   RS_TRY(require_layout(t, layout, &T));
-  if (Q == 0) return RS_OK;
-  if (sc.nseg.bytes < sizeof(int32_t) * (size_t)Q) RS_TRY(sc.nseg.alloc(sizeof(int32_t) * (size_t)Q));
-  if (sc.segoff.bytes < sizeof(int64_t) * (size_t)(Q + 1))
-    RS_TRY(sc.segoff.alloc(sizeof(int64_t) * (size_t)(Q + 1)));
-  const int g = grid_for(Q, 256, 4);
-  ir_seg_count_kernel<<<g, 256, 0, st>>>(t->d_off, t->n, query, qoff, Q, roff,
-                                         sc.nseg.as<int32_t>(), t->d_status());
-  RS_TRY(exclusive_scan_i32_to_i64(sc.nseg.as<int32_t>(), Q, sc.segoff.as<int64_t>(), sc.tmp, st));
-  int64_t total = known_segs;
-  if (sync_total) {
-    RS_HIP(hipMemcpyAsync(&total, sc.segoff.as<int64_t>() + Q, sizeof(int64_t),
-                          hipMemcpyDeviceToHost, st));
-    RS_HIP(hipStreamSynchronize(st));
-  }
-  sc.total_segs = total;
-  if (sc.seg_count.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
-    RS_TRY(sc.seg_count.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
-  if (sc.seg_query.bytes < sizeof(int32_t) * (size_t)(total > 0 ? total : 1))
-    RS_TRY(sc.seg_query.alloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1)));
   if (t->timing) RS_HIP(hipEventRecord(t->ev0, st));
   if (total > 0) {
-    if (!sc.next_run.p) RS_TRY(sc.next_run.alloc(sizeof(unsigned long long)));
-    auto* next_run = sc.next_run.as<unsigned long long>();
-    ir_reset_run_kernel<<<1, 1, 0, st>>>(next_run);
-    ir_seg_map_kernel<<<grid_for(Q, 256, 4), 256, 0, st>>>(sc.segoff.as<int64_t>(), Q,
-                                                          sc.seg_query.as<int32_t>());
-    const int grid = grid_for(total * kWave, kIrBlock, 8);
+    ir_reset_runs_kernel<<<1, 64, 0, st>>>(w.next_run, kIrXcds);
+    ir_seg_map_kernel<<<grid_for(Q, 256, 4), 256, 0, st>>>(w.segoff, Q, w.seg_query);
+    int grid = grid_for(total * kWave, kIrBlock, 8);
     const int64_t nwaves = (int64_t)grid * (kIrBlock / kWave);
     const int64_t steal_min = ir_steal_min >= 0 ? ir_steal_min : 4 * kIrRun * nwaves;
+    const bool steal = total >= steal_min;
+    if (steal) grid = (grid + kIrXcds - 1) / kIrXcds * kIrXcds;
     auto kern = layout == RS_HIERARCHICAL ? ir_segment_kernel<RS_HIERARCHICAL, kIrKeys>
                                           : ir_segment_kernel<RS_NORMAL, kIrKeys>;
-    kern<<<grid, kIrBlock, 0, st>>>(t->d_off, term0_docs, sc.seg_query.as<int32_t>(),
-                                   t->lengths.as<int32_t>(), t->toff.as<int64_t>(), T, t->BN, t->b,
-                                   (uint32_t)t->M, query, qoff, roff, sc.segoff.as<int64_t>(), total,
-                                   ws, sc.seg_count.as<int32_t>(), t->d_status(), next_run, steal_min);
+    kern<<<grid, kIrBlock, 0, st>>>(t->d_off, term0_docs, w.seg_query, t->lengths.as<int32_t>(),
+                                   t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, query, qoff, roff,
+                                   w.segoff, total, ws, w.seg_count, t->d_status(), w.next_run, steal);
+    RS_TRY(exclusive_scan_i32_to_i64_buf(w.seg_count, total, w.seg_pos, w.sums, w.sums_cap, st));
+    ir_copy_kernel<<<grid_for(total * kWave, kIrBlock, 8), kIrBlock, 0, st>>>(
+        w.seg_query, w.segoff, roff, total, ws, w.seg_count, w.seg_pos, result, dense);
+  } else {
+    ir_reset_runs_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(w.seg_pos), 1);
   }
-  ir_gather_kernel<<<grid_for(Q * kWave, kIrBlock, 8), kIrBlock, 0, st>>>(
-      roff, sc.segoff.as<int64_t>(), Q, ws, sc.seg_count.as<int32_t>(), result, count);
+  ir_counts_kernel<<<grid_for(Q, 256, 4), 256, 0, st>>>(w.segoff, Q, w.seg_pos, count, out_off);
   if (t->timing) RS_HIP(hipEventRecord(t->ev1, st));
   RS_HIP(hipGetLastError());
   return RS_OK;
 }
 
-// ---------------------------------------------------------------- server
-// Packs each query's result densely after the batch: out_off[q] = prefix of
-// counts (single block, batches are <= a few thousand queries).
-__global__ void __launch_bounds__(256) ir_pack_kernel(const int64_t* __restrict__ roff,
-                                                      const int32_t* __restrict__ count, int64_t Q,
-                                                      const int32_t* __restrict__ result,
-                                                      int64_t* __restrict__ out_off,
-                                                      int32_t* __restrict__ dense) {
-  __shared__ int64_t carry_s;
-  __shared__ int64_t wsum[4];
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < Q; base += 256) {
-    const int64_t qi = base + threadIdx.x;
-    const int64_t c = qi < Q ? count[qi] : 0;
-    const int64_t inc = wave_inclusive_scan(c);
-    const int wid = threadIdx.x / kWave;
-    if (__lane_id() == kWave - 1) wsum[wid] = inc;
-    __syncthreads();
-    int64_t before = carry_s;
-    for (int w = 0; w < wid; ++w) before += wsum[w];
-    if (qi < Q) out_off[qi] = before + inc - c;
-    __syncthreads();
-    if (threadIdx.x == 0) carry_s += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out_off[Q] = carry_s;
-  __syncthreads();
-  const int lane = __lane_id();
-  for (int64_t qi = threadIdx.x / kWave; qi < Q; qi += 4) {
-    const int64_t s = roff[qi], d = out_off[qi];
-    const int32_t c = count[qi];
-    for (int32_t j = lane; j < c; j += kWave) dense[d + j] = result[s + j];
-  }
+// Sizes the allocating scratch for Q queries / `total` segments (total < 0:
+// phase-1 buffers only) and returns its pointer view.
+static rs_status ir_scratch_view(IrScratch& sc, int64_t Q, int64_t total, IrWork& w) {
+  RS_TRY(grow<int32_t>(sc.nseg, Q));
+  RS_TRY(grow<int64_t>(sc.segoff, Q + 1));
+  const int64_t tiles = scan_tile_count(Q > total ? Q : total) + 1;
+  RS_TRY(grow<int64_t>(sc.sums, tiles));
+  RS_TRY(grow<unsigned long long>(sc.next_run, kIrXcds));
+  w.nseg = sc.nseg.as<int32_t>();
+  w.segoff = sc.segoff.as<int64_t>();
+  w.sums = sc.sums.as<int64_t>();
+  w.sums_cap = (int64_t)(sc.sums.bytes / sizeof(int64_t));
+  w.next_run = sc.next_run.as<unsigned long long>();
+  if (total < 0) return RS_OK;
+  RS_TRY(grow<int32_t>(sc.seg_query, total));
+  RS_TRY(grow<int32_t>(sc.seg_count, total));
+  RS_TRY(grow<int64_t>(sc.seg_pos, total + 1));
+  w.seg_query = sc.seg_query.as<int32_t>();
+  w.seg_count = sc.seg_count.as<int32_t>();
+  w.seg_pos = sc.seg_pos.as<int64_t>();
+  return RS_OK;
 }
 
+// Workspace-carrying enqueue used by the C entry point and by the server:
+// with sync_total the total segment count is read back (one synchronization),
+// else known_segs (the server computes it on the host).
+static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* term0_docs,
+                            const int32_t* query, const int32_t* qoff, int64_t Q,
+                            const int64_t* roff, int32_t* result, int32_t* count, int32_t* ws,
+                            int32_t* dense, int64_t* out_off, IrScratch& sc, hipStream_t st,
+                            bool sync_total, int64_t known_segs) {
+  const int32_t* T = nullptr;
+  RS_TRY(require_layout(t, layout, &T));
+  if (Q == 0) return RS_OK;
+  IrWork w;
+  RS_TRY(ir_scratch_view(sc, Q, -1, w));
+  RS_TRY(ir_plan_segments(t, query, qoff, Q, roff, w, st));
+  int64_t total = known_segs;
+  if (sync_total) {
+    RS_HIP(hipMemcpyAsync(&total, w.segoff + Q, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
+  sc.total_segs = total;
+  RS_TRY(ir_scratch_view(sc, Q, total, w));
+  return ir_run_segments(t, layout, term0_docs, query, qoff, Q, roff, result, count, ws, dense, out_off,
+                         total, w, st);
+}
+
+// ---------------------------------------------------------------- server
 static double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
@@ -338,12 +395,25 @@ struct IrPending {
   int64_t n0;  // |term0 list|
 };
 
+// Bytes an IR query holds while admitted: its term0 entries twice (survivor
+// workspace and dense result, 4 bytes each) plus its offsets and terms.
+static int64_t ir_footprint(const IrPending& p) { return 8 * p.n0 + 64 + 4 * (int64_t)p.terms.size(); }
+
+
+// Terms a served query may have (sizes the slots' preallocated query arrays).
+constexpr int32_t kServeMaxTerms = 256;
+
+// One in-flight batch of the serving loop: its stream, events and every
+// buffer it needs, allocated once by rs_server_create for max_batch queries
+// of up to kServeMaxTerms terms and cap_entries term0 entries in total (the
+// slot's share of the memory budget), so the loop itself never allocates.
 struct IrSlot {
   hipStream_t stream = nullptr;
   hipEvent_t ev_counts = nullptr, ev_done = nullptr;
   int state = 0;  // 0 free, 1 waiting for counts, 2 waiting for results
   std::vector<IrPending> batch;
   int64_t bytes = 0;
+  int64_t cap_entries = 0, cap_segs = 0;
   // host staging (pinned)
   int32_t* h_query = nullptr;
   int32_t* h_qoff = nullptr;
@@ -351,10 +421,10 @@ struct IrSlot {
   int32_t* h_count = nullptr;
   int64_t* h_outoff = nullptr;
   int32_t* h_dense = nullptr;
-  int64_t cap_q = 0, cap_terms = 0, cap_dense = 0;
   // device
-  DevBuf d_query, d_qoff, d_roff, d_result, d_ws, d_count, d_outoff, d_dense;
+  DevBuf d_query, d_qoff, d_roff, d_ws, d_count, d_outoff, d_dense;
   IrScratch sc;
+  IrWork w;
 };
 
 struct rs_server {
@@ -362,6 +432,7 @@ struct rs_server {
   int32_t layout = 0;
   int device = 0;
   int64_t budget = 0;
+  int64_t slot_budget = 0;  // budget / max_inflight: what one slot's buffers hold
   int32_t max_batch = 0;
   std::vector<int32_t> h_idx_off;  // host copy of the index offsets
   std::vector<std::unique_ptr<IrSlot>> slots;
@@ -399,49 +470,41 @@ static rs_status host_alloc(T*& p, int64_t n) {
   return RS_OK;
 }
 
-// Grows a slot's pinned staging and device buffers (doubling) to hold a batch
-// of Q queries with `nterms` terms and `total_n0` term0 entries.
-static rs_status slot_reserve(IrSlot& s, int64_t Q, int64_t nterms, int64_t total_n0) {
-  if (Q + 1 > s.cap_q) {
-    const int64_t c = 2 * (Q + 1);
-    RS_TRY(host_alloc(s.h_qoff, c));
-    RS_TRY(host_alloc(s.h_roff, c));
-    RS_TRY(host_alloc(s.h_count, c));
-    RS_TRY(host_alloc(s.h_outoff, c));
-    RS_TRY(s.d_qoff.alloc(sizeof(int32_t) * (size_t)c));
-    RS_TRY(s.d_roff.alloc(sizeof(int64_t) * (size_t)c));
-    RS_TRY(s.d_count.alloc(sizeof(int32_t) * (size_t)c));
-    RS_TRY(s.d_outoff.alloc(sizeof(int64_t) * (size_t)c));
-    s.cap_q = c;
-  }
-  const int64_t nt = nterms > 0 ? nterms : 1;
-  if (nt > s.cap_terms) {
-    const int64_t c = 2 * nt;
-    RS_TRY(host_alloc(s.h_query, c));
-    RS_TRY(s.d_query.alloc(sizeof(int32_t) * (size_t)c));
-    s.cap_terms = c;
-  }
-  const int64_t need = total_n0 > 0 ? total_n0 : 1;
-  if (need > s.cap_dense) {
-    const int64_t c = 2 * need;
-    RS_TRY(host_alloc(s.h_dense, c));
-    RS_TRY(s.d_result.alloc(sizeof(int32_t) * (size_t)c));
-    RS_TRY(s.d_ws.alloc(sizeof(int32_t) * (size_t)c));
-    RS_TRY(s.d_dense.alloc(sizeof(int32_t) * (size_t)c));
-    s.cap_dense = c;
-  }
-  return RS_OK;
+// Allocates a slot's pinned staging and device buffers (once, at creation).
+static rs_status slot_init(IrSlot& s, int64_t max_batch, int64_t cap_entries) {
+  const int64_t nq = max_batch + 1, nt = max_batch * kServeMaxTerms;
+  s.cap_entries = cap_entries;
+  s.cap_segs = cap_entries / kIrSeg + max_batch;
+  RS_TRY(host_alloc(s.h_qoff, nq));
+  RS_TRY(host_alloc(s.h_roff, nq));
+  RS_TRY(host_alloc(s.h_count, nq));
+  RS_TRY(host_alloc(s.h_outoff, nq));
+  RS_TRY(host_alloc(s.h_query, nt));
+  RS_TRY(host_alloc(s.h_dense, cap_entries));
+  RS_TRY(s.d_qoff.alloc(sizeof(int32_t) * (size_t)nq));
+  RS_TRY(s.d_roff.alloc(sizeof(int64_t) * (size_t)nq));
+  RS_TRY(s.d_count.alloc(sizeof(int32_t) * (size_t)nq));
+  RS_TRY(s.d_outoff.alloc(sizeof(int64_t) * (size_t)nq));
+  RS_TRY(s.d_query.alloc(sizeof(int32_t) * (size_t)nt));
+  RS_TRY(s.d_ws.alloc(sizeof(int32_t) * (size_t)cap_entries));
+  RS_TRY(s.d_dense.alloc(sizeof(int32_t) * (size_t)cap_entries));
+  return ir_scratch_view(s.sc, max_batch, s.cap_segs, s.w);
 }
 
-// Launch a batch on a slot: H2D query arrays, query kernels, pack, D2H counts.
+// Launch a batch on a slot: H2D query arrays, the query kernels writing the
+// results densely (query order), D2H counts and offsets.  Within the slot's
+// preallocated capacity by admission.
 static rs_status slot_launch(rs_server* S, IrSlot& s) {
   const int64_t Q = (int64_t)s.batch.size();
-  int64_t nterms = 0, total = 0;
+  int64_t nterms = 0, total = 0, segs = 0;
   for (auto& p : s.batch) {
     nterms += (int64_t)p.terms.size();
     total += p.n0;
+    segs += (p.n0 + kIrSeg - 1) / kIrSeg;
   }
-  RS_TRY(slot_reserve(s, Q, nterms, total));
+  RS_REQUIRE(Q <= S->max_batch && nterms <= (int64_t)S->max_batch * kServeMaxTerms &&
+                 total <= s.cap_entries && segs <= s.cap_segs,
+             "ir server: batch exceeds the slot's preallocated capacity");
   int64_t w = 0, r = 0;
   for (int64_t i = 0; i < Q; ++i) {
     s.h_qoff[i] = (int32_t)w;
@@ -452,20 +515,15 @@ static rs_status slot_launch(rs_server* S, IrSlot& s) {
   s.h_qoff[Q] = (int32_t)w;
   s.h_roff[Q] = r;
   hipStream_t st = s.stream;
-  RS_HIP(hipMemcpyAsync(s.d_query.p, s.h_query, sizeof(int32_t) * (size_t)(nterms > 0 ? nterms : 1),
-                        hipMemcpyHostToDevice, st));
-  RS_HIP(hipMemcpyAsync(s.d_qoff.p, s.h_qoff, sizeof(int32_t) * (size_t)(Q + 1),
-                        hipMemcpyHostToDevice, st));
-  RS_HIP(hipMemcpyAsync(s.d_roff.p, s.h_roff, sizeof(int64_t) * (size_t)(Q + 1),
-                        hipMemcpyHostToDevice, st));
-  int64_t segs = 0;
-  for (auto& p : s.batch) segs += (p.n0 + kIrSeg - 1) / kIrSeg;
-  RS_TRY(ir_enqueue(S->t, S->layout, S->t->d_elems, s.d_query.as<int32_t>(), s.d_qoff.as<int32_t>(),
-                    Q, s.d_roff.as<int64_t>(), s.d_result.as<int32_t>(), s.d_count.as<int32_t>(),
-                    s.d_ws.as<int32_t>(), s.sc, st, false, segs));
-  ir_pack_kernel<<<1, 256, 0, st>>>(s.d_roff.as<int64_t>(), s.d_count.as<int32_t>(), Q,
-                                    s.d_result.as<int32_t>(), s.d_outoff.as<int64_t>(),
-                                    s.d_dense.as<int32_t>());
+  if (nterms > 0)
+    RS_HIP(hipMemcpyAsync(s.d_query.p, s.h_query, sizeof(int32_t) * (size_t)nterms, hipMemcpyHostToDevice, st));
+  RS_HIP(hipMemcpyAsync(s.d_qoff.p, s.h_qoff, sizeof(int32_t) * (size_t)(Q + 1), hipMemcpyHostToDevice, st));
+  RS_HIP(hipMemcpyAsync(s.d_roff.p, s.h_roff, sizeof(int64_t) * (size_t)(Q + 1), hipMemcpyHostToDevice, st));
+  RS_TRY(ir_plan_segments(S->t, s.d_query.as<int32_t>(), s.d_qoff.as<int32_t>(), Q, s.d_roff.as<int64_t>(),
+                          s.w, st));
+  RS_TRY(ir_run_segments(S->t, S->layout, S->t->d_elems, s.d_query.as<int32_t>(), s.d_qoff.as<int32_t>(), Q,
+                         s.d_roff.as<int64_t>(), nullptr, s.d_count.as<int32_t>(), s.d_ws.as<int32_t>(),
+                         s.d_dense.as<int32_t>(), s.d_outoff.as<int64_t>(), segs, s.w, st));
   RS_HIP(hipMemcpyAsync(s.h_count, s.d_count.p, sizeof(int32_t) * (size_t)Q, hipMemcpyDeviceToHost, st));
   RS_HIP(hipMemcpyAsync(s.h_outoff, s.d_outoff.p, sizeof(int64_t) * (size_t)(Q + 1),
                         hipMemcpyDeviceToHost, st));
@@ -535,9 +593,10 @@ static void server_loop(rs_server* S) {
         int64_t bytes = 0;
         while (!S->pending.empty() && (int32_t)s.batch.size() < S->max_batch) {
           const IrPending& p = S->pending.front();
-          const int64_t fb = 8 * p.n0 + 64 + 4 * (int64_t)p.terms.size();
-          if (S->admitted_bytes + bytes + fb > S->budget && !(s.batch.empty() && S->admitted_bytes == 0))
-            break;
+          const int64_t fb = ir_footprint(p);
+          // the slot's share of the budget (what its buffers hold), and the
+          // budget as a whole
+          if (bytes + fb > S->slot_budget || S->admitted_bytes + bytes + fb > S->budget) break;
           bytes += fb;
           s.batch.push_back(p);
           S->pending.pop_front();
@@ -697,8 +756,8 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
   hipStream_t st = as_stream(stream);
   IrScratch sc;
   RS_TRY(ir_enqueue(t, layout, d_term0_docs ? d_term0_docs : t->d_elems, d_query, d_query_offsets,
-                    num_queries, d_result_offsets, d_result, d_result_count, d_workspace, sc, st,
-                    true, 0));
+                    num_queries, d_result_offsets, d_result, d_result_count, d_workspace, nullptr, nullptr,
+                    sc, st, true, 0));
   RS_HIP(hipStreamSynchronize(st));  // scratch is released on return
   return RS_OK;
 }
@@ -712,8 +771,10 @@ rs_status rs_ir_set_steal_threshold(int64_t segments) {
 rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_budget_bytes,
                               int32_t max_batch, int32_t max_inflight, rs_server** out) {
   RS_REQUIRE(t && out, "rs_server_create: null argument");
-  RS_REQUIRE(memory_budget_bytes > 0 && max_batch > 0 && max_inflight > 0 && max_inflight <= 16,
-             "rs_server_create: bad limits");
+  RS_REQUIRE(memory_budget_bytes > 0 && max_batch > 0 && max_batch <= (1 << 20) && max_inflight > 0 &&
+                 max_inflight <= 16 && memory_budget_bytes / max_inflight >= 4096,
+             "rs_server_create: bad limits (budget %lld bytes, batch %d, inflight %d)",
+             (long long)memory_budget_bytes, max_batch, max_inflight);
   const int32_t* T = nullptr;
   RS_TRY(require_layout(t, layout, &T));
   *out = nullptr;
@@ -730,15 +791,24 @@ rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_bu
     set_error("rs_server_create: cannot read index offsets");
     return RS_EDEVICE;
   }
+  S->slot_budget = memory_budget_bytes / max_inflight;
+  const int64_t cap_entries = S->slot_budget / 8;
   for (int i = 0; i < max_inflight; ++i) S->slots.push_back(std::make_unique<IrSlot>());
   for (auto& sp : S->slots) {
     IrSlot& s = *sp;
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s.ev_counts, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming) != hipSuccess) {
-      delete S;
+      rs_server_free(S);
       set_error("rs_server_create: stream/event creation failed");
       return RS_EDEVICE;
+    }
+    const rs_status st = slot_init(s, max_batch, cap_entries);
+    if (st != RS_OK) {
+      const std::string msg = last_error();
+      rs_server_free(S);
+      set_error("rs_server_create: %s", msg.c_str());
+      return st;
     }
   }
   S->worker = std::thread(server_loop, S);
@@ -756,6 +826,13 @@ rs_status rs_server_submit_ir(rs_server* S, int64_t query_id, const int32_t* h_t
   for (int32_t x : p.terms)
     RS_REQUIRE(x >= 0 && x < S->t->n, "rs_server_submit: term id %d out of range", x);
   p.n0 = num_terms > 0 ? S->h_idx_off[p.terms[0] + 1] - S->h_idx_off[p.terms[0]] : 0;
+  RS_REQUIRE(num_terms <= kServeMaxTerms, "rs_server_submit: a query has more than %d terms", kServeMaxTerms);
+  if (ir_footprint(p) > S->slot_budget) {
+    set_error("rs_server_submit: query %lld needs %lld bytes, above a slot's share of the memory "
+              "budget (%lld bytes)", (long long)query_id, (long long)ir_footprint(p),
+              (long long)S->slot_budget);
+    return RS_ENOMEM;
+  }
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->pending.size() >= (size_t)1 << 22) {

@@ -212,17 +212,37 @@ def test_ir_server_results_alone_and_in_the_mix(api, oracle_lib):
 
 
 def test_ir_server_memory_admission_splits_batches(api, oracle_lib):
-    """A budget below one big query still admits it alone (never deadlocks),
-    and small budgets force many launches without changing results."""
+    """Admission by memory footprint (8 bytes per term0 entry plus offsets):
+    a slot's share of the budget that holds only a few queries forces many
+    launches without changing results, and a query above a slot's share is
+    refused at submit (RS_ENOMEM) while the others are served."""
+    from reset_amd import serving
+    from reset_amd.lib import RS_ENOMEM, ResetError
+
     rng = np.random.default_rng(23)
     idx_off, docs, q, qoff = _ir_case(rng, nterms=200, ndocs=300000, maxlen=40000, nq=120, kmax=4)
     queries = [q[qoff[i]:qoff[i + 1]].tolist() for i in range(120)]
     m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
     want = oracle_lib.ir_result_lists(m_res, m_cnt, m_roff)
+    df = np.diff(idx_off)
+    fp = [8 * int(df[x[0]]) + 64 + 4 * len(x) if x else 64 for x in queries]
     t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 1)
-    got, comps, stats = _serve(api, t, queries, budget=64 << 10, max_batch=16, inflight=2)
+    slot = max(fp) + 1024
+    got, comps, stats = _serve(api, t, queries, budget=2 * slot, max_batch=16, inflight=2)
     assert [got[i] for i in range(120)] == want
-    assert stats["launches"] >= 20
+    assert stats["launches"] >= 20 and stats["peak_bytes"] <= 2 * slot
+    # a slot share below the largest query: that query is refused, the rest served
+    big = int(np.argmax(fp))
+    srv = serving.IRServer(t, budget_bytes=2 * (fp[big] - 1), max_batch=16, max_inflight=2)
+    with pytest.raises(ResetError) as e:
+        srv.submit(big, queries[big])
+    assert e.value.status == RS_ENOMEM
+    small = [i for i in range(120) if fp[i] <= fp[big] - 1][:30]
+    for i in small:
+        srv.submit(i, queries[i])
+    srv.drain()
+    assert all(srv.result(i) == want[i] for i in small)
+    srv.close()
     t.close()
 
 
