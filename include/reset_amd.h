@@ -240,6 +240,23 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
                             int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
                             rs_stream stream);
 
+/* rs_ir_query_async without allocation.  No counterpart in the reference:
+ * this entry point is for callers that cannot allocate device memory (tight
+ * serving loops, code that forbids hipMalloc).  Same arguments, results and
+ * errors as rs_ir_query_async; its scratch (segment offsets, counts, scan
+ * sums and work counters, about 4.8 MB) is a fixed array in the library's
+ * device memory, one per GPU, held until the process ends.  Limits per call:
+ * at most 65,536 queries and 262,144 term0 segments of 4,096 entries
+ * (RS_EINVAL above them; the segment count is checked after one
+ * synchronization of `stream`, and nothing is written then).  Calls that
+ * overlap on one GPU race on that array: issue them on one stream or one
+ * after another.  Returns once the result kernels are enqueued. */
+rs_status rs_ir_query_noalloc(const rs_tables* t, int32_t layout, const int32_t* d_term0_docs,
+                              const int32_t* d_query, const int32_t* d_query_offsets,
+                              int64_t num_queries, const int64_t* d_result_offsets,
+                              int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
+                              rs_stream stream);
+
 /* Batches of at least `segments` term0 segments (4096 entries each) are
  * split among waves by work stealing, smaller ones statically; -1 (the
  * default) means four runs of four segments per wave of the grid.  Results do

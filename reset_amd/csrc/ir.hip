@@ -378,6 +378,62 @@ static rs_status ir_enqueue(const rs_tables* t, int32_t layout, const int32_t* t
                          total, w, st);
 }
 
+// ---------------------------------------------------------------- no allocation
+// rs_ir_query_noalloc: the query scratch in a fixed device array, sized for
+// kNoAllocQueries queries and kNoAllocSegments term0 segments per call.  The
+// array's name stands for a host-side symbol: host code reaches its device
+// address only through hipGetSymbolAddress, once per device (no_alloc_base).
+constexpr int64_t kNoAllocQueries = 1 << 16;
+constexpr int64_t kNoAllocSegments = 1 << 18;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr int64_t kNoAllocSums = (kNoAllocSegments + 2047) / 2048 + 2;  // scan tiles of 2048
+constexpr size_t kNaNseg = 0;
+constexpr size_t kNaSegoff = kNaNseg + align256(sizeof(int32_t) * kNoAllocQueries);
+constexpr size_t kNaSums = kNaSegoff + align256(sizeof(int64_t) * (kNoAllocQueries + 1));
+constexpr size_t kNaSegQuery = kNaSums + align256(sizeof(int64_t) * kNoAllocSums);
+constexpr size_t kNaSegCount = kNaSegQuery + align256(sizeof(int32_t) * kNoAllocSegments);
+constexpr size_t kNaSegPos = kNaSegCount + align256(sizeof(int32_t) * kNoAllocSegments);
+constexpr size_t kNaNextRun = kNaSegPos + align256(sizeof(int64_t) * (kNoAllocSegments + 1));
+constexpr size_t kNoAllocBytes = kNaNextRun + align256(sizeof(unsigned long long) * kIrXcds);
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (with it current) and kept per device.
+static rs_status no_alloc_base(unsigned char** out) {
+  static std::mutex mu;
+  static unsigned char* base[64] = {nullptr};
+  int dev = 0;
+  RS_HIP(hipGetDevice(&dev));
+  RS_REQUIRE(dev >= 0 && dev < 64, "rs_ir_query_noalloc: device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(mu);
+  if (!base[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("rs_ir_query_noalloc: hipGetSymbolAddress of the scratch array failed: %s",
+                hipGetErrorString(e));
+      return RS_EDEVICE;
+    }
+    base[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = base[dev];
+  return RS_OK;
+}
+
+static IrWork no_alloc_view(unsigned char* base) {
+  IrWork w;
+  w.nseg = reinterpret_cast<int32_t*>(base + kNaNseg);
+  w.segoff = reinterpret_cast<int64_t*>(base + kNaSegoff);
+  w.sums = reinterpret_cast<int64_t*>(base + kNaSums);
+  w.sums_cap = kNoAllocSums;
+  w.seg_query = reinterpret_cast<int32_t*>(base + kNaSegQuery);
+  w.seg_count = reinterpret_cast<int32_t*>(base + kNaSegCount);
+  w.seg_pos = reinterpret_cast<int64_t*>(base + kNaSegPos);
+  w.next_run = reinterpret_cast<unsigned long long*>(base + kNaNextRun);
+  return w;
+}
+
 // ---------------------------------------------------------------- server
 static double now_ms() {
   using namespace std::chrono;
@@ -760,6 +816,36 @@ rs_status rs_ir_query_async(const rs_tables* t, int32_t layout, const int32_t* d
                     sc, st, true, 0));
   RS_HIP(hipStreamSynchronize(st));  // scratch is released on return
   return RS_OK;
+}
+
+rs_status rs_ir_query_noalloc(const rs_tables* t, int32_t layout, const int32_t* d_term0_docs,
+                              const int32_t* d_query, const int32_t* d_query_offsets,
+                              int64_t num_queries, const int64_t* d_result_offsets,
+                              int32_t* d_result, int32_t* d_result_count, int32_t* d_workspace,
+                              rs_stream stream) {
+  RS_REQUIRE(t, "rs_ir_query_noalloc: null tables");
+  RS_REQUIRE(num_queries >= 0 && num_queries <= kNoAllocQueries,
+             "rs_ir_query_noalloc: %lld queries (at most %lld per call)", (long long)num_queries,
+             (long long)kNoAllocQueries);
+  if (num_queries == 0) return RS_OK;
+  RS_REQUIRE(d_query_offsets && d_result_offsets && d_result_count && d_result && d_workspace,
+             "rs_ir_query_noalloc: null argument");
+  const int32_t* T = nullptr;
+  RS_TRY(require_layout(t, layout, &T));
+  hipStream_t st = as_stream(stream);
+  unsigned char* base = nullptr;
+  RS_TRY(no_alloc_base(&base));
+  const IrWork w = no_alloc_view(base);
+  const int32_t* term0 = d_term0_docs ? d_term0_docs : t->d_elems;
+  RS_TRY(ir_plan_segments(t, d_query, d_query_offsets, num_queries, d_result_offsets, w, st));
+  int64_t total = 0;
+  RS_HIP(hipMemcpyAsync(&total, w.segoff + num_queries, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  RS_REQUIRE(total <= kNoAllocSegments,
+             "rs_ir_query_noalloc: the batch has %lld term0 segments of %lld entries (at most %lld per "
+             "call)", (long long)total, (long long)kIrSeg, (long long)kNoAllocSegments);
+  return ir_run_segments(t, layout, term0, d_query, d_query_offsets, num_queries, d_result_offsets, d_result,
+                         d_result_count, d_workspace, nullptr, nullptr, total, w, st);
 }
 
 rs_status rs_ir_set_steal_threshold(int64_t segments) {

@@ -90,6 +90,7 @@ SIGNATURES = {
     "rs_sss_index_free": (None, [_p]),
     "rs_sss_lookup": (_i32, [_p, _p, _i64, _p, _p, _pp]),
     "rs_ir_query_async": (_i32, [_p, _i32, _p, _p, _p, _i64, _p, _p, _p, _p, _p]),
+    "rs_ir_query_noalloc": (_i32, [_p, _i32, _p, _p, _p, _i64, _p, _p, _p, _p, _p]),
     "rs_server_create": (_i32, [_p, _i32, _i64, _i32, _i32, _pp]),
     "rs_server_submit_ir": (_i32, [_p, _i64, _p, _i32]),
     "rs_server_attach_sss": (_i32, [_p, _p]),
