@@ -74,6 +74,61 @@ inline int read_int_binary(const std::string& filename) {
   if (!in) throw std::runtime_error("Truncated file: " + filename);
   return value;
 }
+// The 64-bit dataset files of the multi-GPU drivers (multi-GPU/sss-multi-GPU/
+// main.cu:25-57): a little-endian uint64 count followed by int64 values, and a
+// scalar file holding one int64.  read_vec_64 / read_num_64 narrow to int as
+// the reference does, but a value outside the int32 range throws instead of
+// wrapping.
+inline std::vector<int64_t> read_i64_vec(const std::string& filename) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in) throw std::runtime_error("Error opening file: " + filename);
+  uint64_t size = 0;
+  in.read(reinterpret_cast<char*>(&size), sizeof(size));
+  if (!in) throw std::runtime_error("Truncated file: " + filename);
+  std::vector<int64_t> vec(size);
+  in.read(reinterpret_cast<char*>(vec.data()), (std::streamsize)(size * sizeof(int64_t)));
+  if (!in) throw std::runtime_error("Truncated file: " + filename);
+  return vec;
+}
+
+inline int narrow_i32(int64_t x, const std::string& filename) {
+  if (x < INT32_MIN || x > INT32_MAX)
+    throw std::out_of_range("Value " + std::to_string(x) + " of " + filename + " does not fit int32");
+  return (int)x;
+}
+
+inline std::vector<int> read_vec_64(const std::string& filename) {
+  const std::vector<int64_t> wide = read_i64_vec(filename);
+  std::vector<int> vec(wide.size());
+  for (size_t i = 0; i < wide.size(); ++i) vec[i] = narrow_i32(wide[i], filename);
+  return vec;
+}
+
+inline int read_num_64(const std::string& filename) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in) throw std::runtime_error("Error opening file: " + filename);
+  int64_t value = 0;
+  in.read(reinterpret_cast<char*>(&value), sizeof(value));
+  if (!in) throw std::runtime_error("Truncated file: " + filename);
+  return narrow_i32(value, filename);
+}
+
+inline void write_vec_64(const std::string& filename, const std::vector<int64_t>& v) {
+  std::ofstream out(filename, std::ios::binary);
+  if (!out) throw std::runtime_error("Error opening file: " + filename);
+  const uint64_t size = v.size();
+  out.write(reinterpret_cast<const char*>(&size), sizeof(size));
+  out.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(size * sizeof(int64_t)));
+  if (!out) throw std::runtime_error("Write failed: " + filename);
+}
+
+inline void write_num_64(const std::string& filename, int64_t value) {
+  std::ofstream out(filename, std::ios::binary);
+  if (!out) throw std::runtime_error("Error opening file: " + filename);
+  out.write(reinterpret_cast<const char*>(&value), sizeof(value));
+  if (!out) throw std::runtime_error("Write failed: " + filename);
+}
+
 inline void write_i32_vec(const std::string& filename, const std::vector<int>& v) {
   std::ofstream out(filename, std::ios::binary);
   if (!out) throw std::runtime_error("Error opening file: " + filename);

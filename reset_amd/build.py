@@ -34,6 +34,9 @@ HIP_FLAGS = [
 SOURCES = ["core.hip", "tables.hip", "tc.hip", "pairs.hip", "sss.hip", "sss_join.hip", "sss_group.hip", "ir.hip",
            "gen.hip", "autotune.hip", "solver.hip", "competitors.hip"]
 TOOLS = ["tc", "sss", "ir", "intersection"]
+# the standalone apps' make_hash_table, one object per app (their globals differ)
+STANDALONE_DIR = os.path.join(HERE, "standalone")
+STANDALONE_APPS = {"tc": "RESET_AMD_APP_TC", "sss": "RESET_AMD_APP_SSS", "ir": "RESET_AMD_APP_IR"}
 
 
 def _run(cmd):
@@ -72,7 +75,22 @@ def build_tools(jobs: int = 4) -> list:
         return out
 
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(todo) or 1))) as ex:
-        return list(ex.map(one, todo))
+        out = list(ex.map(one, todo))
+    # make_hash_table objects, and a TC main in the free-function form linked with one
+    os.makedirs(STANDALONE_DIR, exist_ok=True)
+    src = os.path.join(CSRC, "standalone", "make_hash_table.cpp")
+
+    def obj(app):
+        o = os.path.join(STANDALONE_DIR, f"{app}_hashtable.o")
+        _run([HIPCC, *HIP_FLAGS, "-x", "hip", f"-D{STANDALONE_APPS[app]}", "-c", src, "-o", o])
+        return o
+
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        objs = dict(zip(STANDALONE_APPS, ex.map(obj, STANDALONE_APPS)))
+    exe = os.path.join(TOOLS_DIR, "standalone_tc")
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", os.path.join(cli, "standalone_tc.cpp"), "-x", "none", objs["tc"],
+          "-o", exe, f"-L{HERE}", "-lreset_amd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
+    return out + [exe]
 
 
 def main(argv=None):

@@ -691,9 +691,10 @@ static void server_loop(rs_server* S) {
   }
 }
 
-// Footprint charged to a lookup while it is admitted: candidate keys, pair
-// sizes and staging for up to 64 candidates per query token.
-static int64_t sss_footprint(const IrPending& p) { return 1024 + 64 * 16 * (int64_t)p.terms.size(); }
+// Footprint charged to a lookup while it is admitted: its tokens, and 32
+// bytes for each candidate it can produce (key, pair ids, size and flag, sort
+// space), from the index's bound computed at submit (kept in n0).
+static int64_t sss_footprint(const IrPending& p) { return 1024 + 8 * (int64_t)p.terms.size() + 32 * p.n0; }
 
 // One lookup batch: stage the query sets, run the lookup on the server's
 // lookup stream (the pipeline synchronizes that stream between its stages),
@@ -959,13 +960,13 @@ rs_status rs_server_submit_sss(rs_server* S, int64_t query_id, const int32_t* h_
   p.id = query_id;
   p.submit_ms = now_ms();
   p.terms.assign(h_tokens, h_tokens + num_tokens);
-  p.n0 = 0;
   for (int32_t i = 0; i < num_tokens; ++i)
     RS_REQUIRE(p.terms[i] >= 0 && (i == 0 || p.terms[i - 1] < p.terms[i]),
                "rs_server_submit_sss: tokens must be non-negative and strictly ascending");
   {
     std::lock_guard<std::mutex> lk(S->mu);
     RS_REQUIRE(S->sss, "rs_server_submit_sss: no similarity index attached");
+    p.n0 = sss_candidate_bound(S->sss, p.terms.data(), num_tokens);
     if (S->sss_pending.size() >= (size_t)1 << 22) {
       set_error("rs_server_submit_sss: pending queue full");
       return RS_EBUSY;

@@ -15,7 +15,10 @@
 //   5. candidate keys (i << 32 | j), radix sort + unique
 //   6. hash verification of every candidate (exact integer rule), compaction
 // Limits: < 2^21 sets, sets < 2^21 tokens, token ids < 2^22.
+#include <algorithm>
 #include <cstring>
+#include <vector>
+
 #include <rocprim/rocprim.hpp>
 
 #include "sss_verify.hpp"
@@ -27,6 +30,9 @@ struct rs_sss_index {
   int32_t p = 0, q = 0;
   rs::DevBuf rank;  // int32[V]: global rank of each token id
   rs::DevBuf ent;   // uint64[M]: sorted prefix entries rank << 42 | size << 21 | set
+  // host copies for admission: the rank of each token id, and the number of
+  // prefix entries of each rank
+  std::vector<int32_t> h_rank, h_rank_entries;
 };
 
 struct rs_sss_result {
@@ -98,7 +104,7 @@ __global__ void map_rank_kernel(const int32_t* __restrict__ tok, int64_t n,
     out[i] = rank[tok[i]];
 }
 
-__device__ __forceinline__ int32_t prefix_len(int32_t s, int32_t p, int32_t q) {
+__host__ __device__ __forceinline__ int32_t prefix_len(int32_t s, int32_t p, int32_t q) {
   int64_t pl = (int64_t)s - ((int64_t)p * s + q - 1) / q + 1;
   if (pl > s) pl = s;
   return pl < 0 ? 0 : (int32_t)pl;
@@ -702,6 +708,33 @@ static rs_status verify_candidates(const int32_t* a_off, const int32_t* a_el, co
   return RS_OK;
 }
 
+// cnt[r] = prefix entries of rank r (entries are sorted, rank in the top bits).
+__global__ void rank_entries_kernel(const uint64_t* __restrict__ ent, int64_t M, int64_t V,
+                                    int32_t* __restrict__ cnt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = (int64_t)(ent[i] >> kRankShift);
+    if (r < V) atomicAdd(&cnt[r], 1);
+  }
+}
+
+int64_t sss_candidate_bound(const rs_sss_index* X, const int32_t* h_tokens, int32_t n) {
+  // the lookup's candidates of a query are the entries sharing a rank with
+  // one of its prefix tokens (ranks ascending, tokens outside the collection
+  // first and without entries), before the size window and deduplication
+  std::vector<int64_t> keys((size_t)n);
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t t = h_tokens[i];
+    keys[(size_t)i] = (t >= 0 && t < (int64_t)X->h_rank.size()) ? (int64_t)X->h_rank[(size_t)t] + 1 : 0;
+  }
+  std::sort(keys.begin(), keys.end());
+  const int32_t pl = n > 0 ? prefix_len(n, X->p, X->q) : 0;
+  int64_t bound = 0;
+  for (int32_t i = 0; i < pl && i < n; ++i)
+    if (keys[(size_t)i] > 0) bound += X->h_rank_entries[(size_t)(keys[(size_t)i] - 1)];
+  return bound;
+}
+
 static rs_status index_build(const int32_t* d_off, int64_t n, const int32_t* d_el, int32_t p,
                              int32_t q, hipStream_t st, rs_sss_index* X) {
   RS_REQUIRE(n >= 0 && n < (1ll << 21), "rs_sss_index_build: %lld sets exceed 2^21", (long long)n);
@@ -724,6 +757,21 @@ static rs_status index_build(const int32_t* d_off, int64_t n, const int32_t* d_e
   RS_TRY(token_ranks(d_el, X->E, tmp, st, scratch.as<int32_t>(), X->rank, &X->V));
   RS_TRY(ranked_sets(d_off, n, d_el, X->E, X->rank, X->V, tmp, st, ranked_sorted));
   RS_TRY(prefix_entries(d_off, n, ranked_sorted, p, q, tmp, st, scratch.as<int32_t>(), X->ent, &X->M));
+  // host copies for the serving loop's admission bound
+  X->h_rank.resize((size_t)X->V);
+  X->h_rank_entries.assign((size_t)X->V, 0);
+  if (X->V > 0) {
+    DevBuf cnt;
+    RS_TRY(cnt.alloc(sizeof(int32_t) * (size_t)X->V));
+    RS_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (size_t)X->V, st));
+    if (X->M > 0)
+      rank_entries_kernel<<<grid_for(X->M, 256, 8), 256, 0, st>>>(X->ent.as<uint64_t>(), X->M, X->V,
+                                                                  cnt.as<int32_t>());
+    RS_HIP(hipMemcpyAsync(X->h_rank.data(), X->rank.p, sizeof(int32_t) * (size_t)X->V, hipMemcpyDeviceToHost, st));
+    RS_HIP(hipMemcpyAsync(X->h_rank_entries.data(), cnt.p, sizeof(int32_t) * (size_t)X->V,
+                          hipMemcpyDeviceToHost, st));
+    RS_HIP(hipStreamSynchronize(st));
+  }
   RS_HIP(hipStreamSynchronize(st));
   return RS_OK;
 }

@@ -36,6 +36,11 @@ struct rs_tables {
 };
 
 namespace rs {
+// Upper bound on the candidates a similarity lookup of these tokens produces
+// against the index (sss_join.hip): prefix entries sharing a rank with one of
+// the query's prefix tokens.
+int64_t sss_candidate_bound(const rs_sss_index* X, const int32_t* h_tokens, int32_t n);
+
 // Checks that `layout` was built and returns the table or sets an error.
 rs_status require_layout(const rs_tables* t, int32_t layout, const int32_t** table);
 }  // namespace rs

@@ -115,3 +115,23 @@ def test_ir_cli_matches_chained_merge(tmp_path, oracle_lib):
 def test_intersection_cli_forms_agree(na, nb):
     out = _run(["intersection", na, nb, 11])
     assert _num(r"Intersection Result Count: (\d+)", out) >= 0
+
+
+@pytest.mark.parametrize("flags", [[], ["--alpha=0.5", "--bucket=2"]])
+def test_standalone_make_hash_table_form(tmp_path, oracle_lib, flags):
+    """A TC main in the reference's free-function form (globals + make_hash_table,
+    triangle-count/TC.cuh:18-38) linked with tc_hashtable.o: the published
+    device globals are the handle's tables, and both layouts count the
+    oracle's triangles."""
+    from reset_amd import binfmt
+
+    rng = np.random.default_rng(13)
+    n = 2000
+    und = G.undirected(G.random_graph(n, 30000, rng), n)
+    off, cols = G.orient(und, n, "degree")
+    binfmt.write_tc_folder(tmp_path, off, cols)
+    out = _run(["standalone_tc", tmp_path, *flags])
+    want = oracle_lib.tc_merge(off, cols)
+    assert "Globals match the handle: yes" in out
+    assert _num(r"Normal triangle count: (\d+)", out) == want
+    assert _num(r"Hierarchical triangle count: (\d+)", out) == want
