@@ -97,7 +97,7 @@ def bench_pairs(args):
 
     P = args.pairs
     t0 = time.time()
-    batch = inputs.make_pair_batch(P, seed=SEED)
+    batch = inputs.make_pair_batch(P, seed=SEED, lo=args.pair_lo, hi=args.pair_hi)
     _sync_all()
     gen_s = time.time() - t0
     a_off, a, b_off, b, k = batch["a_off"], batch["a"], batch["b_off"], batch["b"], batch["k"]
@@ -107,13 +107,14 @@ def bench_pairs(args):
     opts = api.default_pairs_opts()
     tune = None
     if not args.no_autotune:
-        # per-bin (load factor, chunk) by timing candidates on a sample (not timed)
+        # per-bin (load factor, bucket size, chunk, lanes per pair) by timing
+        # candidates on a sample (not timed)
         S = min(P, args.tune_sample)
         t1 = time.time()
         opts = api.autotune_pairs(a_off[:S + 1].contiguous(), a, b_off[:S + 1].contiguous(), b, opts, reps=2)
         _sync_all()
         tune = {"sample_pairs": S, "tune_s": time.time() - t1,
-                "bins_alpha_b_chunk_unroll": api.pairs_opts_table(opts)}
+                "bins_alpha_b_chunk_unroll_lanes": api.pairs_opts_table(opts)}
     plan = api.PairsPlan(P, opts)
     sizes = torch.empty(P, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
@@ -160,8 +161,9 @@ def bench_pairs(args):
         "dtype": "int32",
         "data": "synthetic (seeded on device: log-uniform sizes, Feistel-permuted 2^24 universe)",
         "config": {
-            "workload": "batched pairwise intersection: 1M set pairs, |A|,|B| log-uniform in [16,65536], "
-                        "sorted unique uint32 from 2^24, overlap U[0,0.5], one persistent size-binned launch",
+            "workload": f"batched pairwise intersection: {P} set pairs, |A|,|B| log-uniform in "
+                        f"[{args.pair_lo},{args.pair_hi}], sorted unique uint32 from 2^24, overlap U[0,0.5], "
+                        "persistent size-binned launches",
             "pairs": P,
             "elements": n_elems,
             "parallelism": "single GPU",
@@ -987,6 +989,8 @@ def main(argv=None):
     ap.add_argument("--sss-qps", type=float, default=500.0)
     ap.add_argument("--sss-sets", type=int, default=200_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--pair-lo", type=int, default=16, help="pairs: smallest set size")
+    ap.add_argument("--pair-hi", type=int, default=65536, help="pairs: largest set size")
     ap.add_argument("--scale", type=int, default=None,
                     help="R-MAT scale: 24 for tc, 20 for compare (whose SSS part uses scale - 2)")
     ap.add_argument("--cpu-sample", type=int, default=50_000,

@@ -333,9 +333,12 @@ void rs_server_free(rs_server* s);
 
 /* ----------------------------------------------------------------------- a12 */
 /* Pairwise set intersection sizes for P pairs (A_i, B_i) given as two CSR
- * families with int64 offsets.  Sets need not be sorted for the hash probe;
- * the batched form requires ascending sets when the smaller set exceeds the
- * bin's chunk (below). */
+ * families with int64 offsets.  Elements within a set must be distinct.
+ * Sets need not be sorted for the hash probe; the batched form requires
+ * ascending sets when the smaller set exceeds the bin's chunk (below), and
+ * reports sets out of order there (RS_EINVAL from rs_pairs_plan_status)
+ * rather than counting them.  Bins with different bucket sizes or lanes per
+ * pair run as consecutive kernels, one per run of equal settings. */
 typedef struct rs_pairs_bin {
   float load_factor;   /* alpha of the per-pair LDS table                  */
   int32_t bucket_size; /* b: slots per bucket, 1, 2 or 4 (one LDS read)    */
@@ -344,6 +347,11 @@ typedef struct rs_pairs_bin {
   int32_t unroll;      /* 16-byte loads per lane in flight (1, 2, 4) where */
                        /* |larger| >= 4 |smaller| (else 1); same in every  */
                        /* bin                                              */
+  int32_t lanes;       /* probe width: lanes per pair, 64 (a wave), 32 or  */
+                       /* 16 (2 or 4 pairs per wave at once, the smaller   */
+                       /* set built whole); below 64 only for a bin other  */
+                       /* than the last whose table of 2^bin - 1 keys fits */
+                       /* 1/(64/lanes) of a wave's LDS region              */
 } rs_pairs_bin;
 #define RS_PAIRS_MAX_BINS 20
 typedef struct rs_pairs_opts {
@@ -425,10 +433,11 @@ rs_status rs_pairs_plan_kernel_ms(rs_pairs_plan* p, float* h_ms);
 /* Device status of the plan's last launches (RS_EOVERFLOW on a full table). */
 rs_status rs_pairs_plan_status(rs_pairs_plan* p, rs_stream stream);
 
-/* Autotuner: times candidate (alpha, b, chunk, unroll) settings per size bin
+/* Autotuner: times candidate (alpha, b, chunk, lanes) settings per size bin
  * on the given pairs (a sample of the workload) and writes the fastest into
- * *opts; the analytic occupancy model of hyperparameter_solver/src/solver.py
- * ranks the candidates it tries first. */
+ * *opts (the unroll stays the caller's); b in {1, 2, 4}, lanes per pair in
+ * {64, 32, 16} where the bin allows it.  The analytic cost model of
+ * hyperparameter_solver/src/solver.py ranks the candidates it tries first. */
 rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_a, const int64_t* d_b_off,
                             const int32_t* d_b, int64_t num_pairs, int32_t reps,
                             rs_pairs_opts* opts, rs_stream stream);

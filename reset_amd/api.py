@@ -388,8 +388,9 @@ def hyper_solve(arr_size, W=32, S_trans=128, S_slot=4):
 
 
 def autotune_pairs(a_off, a, b_off, b, opts=None, reps=2, stream=None):
-    """Per-size-bin (load factor, chunk) picked by timing candidates on these
-    pairs (a sample of the workload); returns the tuned rs_pairs_opts."""
+    """Per-size-bin (load factor, bucket size, chunk, lanes per pair) picked by
+    timing candidates on these pairs (a sample of the workload); returns the
+    tuned rs_pairs_opts."""
     _need_cuda(a_off, a, b_off, b)
     o = opts if opts is not None else default_pairs_opts()
     check(load().rs_pairs_autotune(ptr(a_off), ptr(a), ptr(b_off), ptr(b), a_off.numel() - 1, int(reps),
@@ -398,9 +399,9 @@ def autotune_pairs(a_off, a, b_off, b, opts=None, reps=2, stream=None):
 
 
 def pairs_opts_table(o):
-    """The bins of an rs_pairs_opts as plain tuples (alpha, b, chunk, unroll)."""
-    return [(round(o.bins[i].load_factor, 4), o.bins[i].bucket_size, o.bins[i].chunk, o.bins[i].unroll)
-            for i in range(o.num_bins)]
+    """The bins of an rs_pairs_opts as plain tuples (alpha, b, chunk, unroll, lanes)."""
+    return [(round(o.bins[i].load_factor, 4), o.bins[i].bucket_size, o.bins[i].chunk, o.bins[i].unroll,
+             o.bins[i].lanes) for i in range(o.num_bins)]
 
 
 class PairsPlan:

@@ -1,20 +1,22 @@
-// autotune.hip -- per-size-bin choice of (load factor, bucket size, chunk)
-// for the batched pair kernel, by timing candidates on a sample (the unroll
-// is the caller's, the same in every bin).
+// autotune.hip -- per-size-bin choice of (load factor, bucket size, chunk,
+// lanes per pair) for the batched pair kernel, by timing candidates on a
+// sample (the unroll is the caller's, the same in every bin).
 //
-// Candidates are the settings whose LDS table (calculate_length(chunk) x b
-// slots) fits one wave's 2048-slot region.  They are tried in order of the
-// analytic cost model's estimate (the reference's hyperparameter solver,
-// restated in solver.hip), cheapest first.  Each bin takes the fastest
-// measured one if it beats the bin's current setting by 3%, else keeps that
-// setting.
+// Candidates are the settings with b in {1, 2, 4} whose LDS table
+// (calculate_length(chunk) x b slots) fits one wave's 2048-slot region, in
+// order of the analytic cost model's estimate (the reference's
+// hyperparameter solver, restated in solver.hip), cheapest first; for a bin
+// that allows it, the best few also run on 32 and 16 lanes per pair (the
+// reference's probe width W).  Each bin takes the fastest measured one if it
+// beats the bin's current setting by 3%, else keeps that setting.
 #include <algorithm>
 #include <vector>
 
 #include "common.hpp"
 
 namespace rs {
-rs_status pairs_launch_ordered(const rs_pairs_opts* opts, const int64_t* d_a_off,
+bool pairs_subwave_fits(const rs_pairs_opts* opts, int bin, const rs_pairs_bin* b);
+rs_status pairs_launch_ordered(const rs_pairs_opts* opts, int bin, const int64_t* d_a_off,
                                const int32_t* d_a, const int64_t* d_b_off, const int32_t* d_b,
                                const int32_t* d_order, int64_t count, int32_t* d_ctl,
                                int32_t* d_sizes, hipStream_t st);
@@ -38,8 +40,8 @@ struct Cand {
 // lanes, 128-byte LDS transactions, 4-byte slots.  Computed once.
 const std::vector<Cand>& candidates() {
   static const std::vector<Cand> cached = [] {
-    const float alphas[] = {0.125f, 0.25f, 0.5f, 0.75f, 0.9f};
-    const int bs[] = {4};
+    const float alphas[] = {0.0625f, 0.125f, 0.25f, 0.5f, 0.75f, 0.9f};
+    const int bs[] = {1, 2, 4};
     const int chunks[] = {127, 255, 511, 1024};
     std::vector<Cand> out;
     for (float a : alphas)
@@ -47,7 +49,7 @@ const std::vector<Cand>& candidates() {
         for (int c : chunks) {
           const int32_t L = calc_length(c, a, b);
           if (L <= 0 || (int64_t)L * b > 2048) continue;  // default LDS region
-          out.push_back(Cand{rs_pairs_bin{a, b, c, 0}, rs_hyper_cost(c, a, b, 64, 128, 4)});
+          out.push_back(Cand{rs_pairs_bin{a, b, c, 0, 64}, rs_hyper_cost(c, a, b, 64, 128, 4)});
         }
     // cheapest first; among equal costs the larger chunk (fewer rebuilds)
     std::stable_sort(out.begin(), out.end(), [](const Cand& x, const Cand& y) {
@@ -106,16 +108,24 @@ extern "C" rs_status rs_pairs_autotune(const int64_t* d_a_off, const int32_t* d_
     std::vector<Cand> tries;
     tries.push_back(Cand{best_bin, 0.0});
     tries.insert(tries.end(), cands.begin(), cands.end());
-    for (const Cand& c : tries) {
-      rs_pairs_opts trial = *opts;
-      for (int k = 0; k < nb; ++k) {
-        trial.bins[k] = c.bin;
-        trial.bins[k].unroll = opts->bins[0].unroll;
+    // part-wave probe widths for the bin's current setting and the model's
+    // three cheapest, where the bin allows them
+    const size_t nfull = tries.size();
+    for (size_t k = 0; k < nfull && k < 4; ++k)
+      for (int lanes : {32, 16}) {
+        Cand c = tries[k];
+        c.bin.lanes = lanes;
+        if (pairs_subwave_fits(opts, bin, &c.bin)) tries.push_back(c);
       }
+    for (const Cand& c : tries) {
+      // only this bin's kernel runs; the other bins keep their (valid) settings
+      rs_pairs_opts trial = *opts;
+      trial.bins[bin] = c.bin;
+      trial.bins[bin].unroll = opts->bins[0].unroll;
       float ms_min = 1e30f;
       for (int r = 0; r <= reps && result == RS_OK; ++r) {  // r == 0 warms up
         (void)hipEventRecord(e0, st);
-        result = pairs_launch_ordered(&trial, d_a_off, d_a, d_b_off, d_b, order.as<int32_t>(), cnt,
+        result = pairs_launch_ordered(&trial, bin, d_a_off, d_a, d_b_off, d_b, order.as<int32_t>(), cnt,
                                       ctl.as<int32_t>(), sizes.as<int32_t>(), st);
         (void)hipEventRecord(e1, st);
         (void)hipEventSynchronize(e1);

@@ -14,6 +14,8 @@
 // for the larger set's elements up to its own last element; its elements are
 // staged through LDS ahead of use.  Everything stays integer; the per-pair
 // count is reduced across the wave and written once.
+#include <vector>
+
 #include "common.hpp"
 
 namespace rs {
@@ -88,14 +90,35 @@ struct PairSrcIdx {
 
 // ---------------------------------------------------------------- plan
 // Layout of the plan's control block (int32): [0..kMaxBins) histogram,
-// [kMaxBins..2kMaxBins) cursors, [2kMaxBins] the work-stealing counter,
-// [2kMaxBins+1] error bits.  Reset by a kernel each launch (no memset inside
-// graphs).
+// [kMaxBins..2kMaxBins) cursors, [2kMaxBins..3kMaxBins) the work-stealing
+// counters of the launch groups (one per group of bins), [3kMaxBins] error
+// bits.  Reset by a kernel each launch (no memset inside graphs).
 constexpr int kCtlHist = 0;
 constexpr int kCtlCursor = RS_PAIRS_MAX_BINS;
 constexpr int kCtlNext = 2 * RS_PAIRS_MAX_BINS;
-constexpr int kCtlStatus = 2 * RS_PAIRS_MAX_BINS + 1;
-constexpr int kCtlWords = 2 * RS_PAIRS_MAX_BINS + 2;
+constexpr int kCtlStatus = 3 * RS_PAIRS_MAX_BINS;
+constexpr int kCtlWords = 3 * RS_PAIRS_MAX_BINS + 1;
+
+// The positions of the size-sorted order one launch works on: the bins
+// bin_lo..bin_hi of the plan (read from the control block on the device:
+// after the scatter, cursor[bin] is one past the bin's last position), or
+// [begin, end) when bin_hi < 0.  `counter` is the launch's work-stealing word.
+struct PairRange {
+  int64_t begin, end;
+  int32_t bin_lo, bin_hi;
+  int32_t counter;
+};
+
+__device__ __forceinline__ void pair_range(const PairRange& rg, const int32_t* ctl, int64_t& b,
+                                           int64_t& e) {
+  if (rg.bin_hi < 0) {
+    b = rg.begin;
+    e = rg.end;
+  } else {
+    e = ctl[kCtlCursor + rg.bin_lo];
+    b = (int64_t)ctl[kCtlCursor + rg.bin_hi] - ctl[kCtlHist + rg.bin_hi];
+  }
+}
 
 __global__ void plan_reset(int32_t* ctl) {
   if (threadIdx.x < kCtlWords - 1) ctl[threadIdx.x] = 0;  // keeps the error word
@@ -525,7 +548,7 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
 // pair is processed.
 template <int B, int U, int CPL, class Src>
 __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
-    const Src src, int64_t P, const int32_t* __restrict__ order, int32_t* ctl,
+    const Src src, const PairRange rg, const int32_t* __restrict__ order_all, int32_t* ctl,
     int32_t* __restrict__ sizes, const PairsConst cfg, int32_t slots_per_wave) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_dyn[];
   const int lane = __lane_id();
@@ -539,6 +562,13 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
     *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
   __builtin_amdgcn_wave_barrier();
   const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
+  // this launch's positions: order[0 .. P) below
+  int64_t pb = 0, pe = 0;
+  pair_range(rg, ctl, pb, pe);
+  pb = __builtin_amdgcn_readfirstlane((int)pb);  // (positions are < 2^31)
+  pe = __builtin_amdgcn_readfirstlane((int)pe);
+  const int64_t P = pe > pb ? pe - pb : 0;
+  const int32_t* __restrict__ order = order_all + pb;
   int32_t K = 0;
   if (cfg.steal > 0) {
     const int64_t fit = P / (4 * W);
@@ -548,7 +578,7 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
   // the counter counts runs: at most P / K + 2W takes, below 2^32
   auto take = [&]() -> int64_t {
     uint32_t v = 0;
-    if (lane == 0) v = atomicAdd(reinterpret_cast<uint32_t*>(ctl + kCtlNext), 1u);
+    if (lane == 0) v = atomicAdd(reinterpret_cast<uint32_t*>(ctl + rg.counter), 1u);
     return (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)v) * K;
   };
   // stealing: the run in use starts at q0 (position q0 + r handed out last);
@@ -642,6 +672,91 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
   if (__any(unordered != 0) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrBadInput);
 }
 
+// Small pairs on part of a wave: each group of LANES lanes (64 / LANES of
+// them per wave) takes its own pair, with its own table of slots_per_wave /
+// (64 / LANES) slots in the wave's LDS region (the reference's probe width,
+// its warp width W, hyperparameter_solver/src/config.json:9).  Only for bins
+// whose pairs all fit the resident path: the smaller set S (below 2^bin
+// elements) is inserted by the group's lanes, the larger set G streams
+// through in 16-byte loads, 4 * LANES elements per step, and the group's
+// count is reduced over its lanes.  The table is cleared by a fill of the
+// L * B slots the pair used.  Positions are dealt statically: step k of wave
+// w covers positions (w + k * W) * (64 / LANES) + group.
+template <int B, int LANES, class Src>
+__global__ void __launch_bounds__(kPairBlock) pairs_subwave_kernel(
+    const Src src, const PairRange rg, const int32_t* __restrict__ order_all, int32_t* ctl,
+    int32_t* __restrict__ sizes, float alpha, int32_t slots_per_wave) {
+  constexpr int G = kWave / LANES;
+  extern __shared__ __attribute__((aligned(16))) int32_t lds_dyn[];
+  const int lane = __lane_id();
+  const int grp = lane / LANES, sl = lane % LANES;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int32_t sub_slots = slots_per_wave / G;
+  int32_t* tab = lds_dyn + wid * (slots_per_wave + kRing) + grp * sub_slots;
+  for (int32_t s4 = lane; s4 < (slots_per_wave >> 2); s4 += kWave)
+    *reinterpret_cast<v4i*>(lds_dyn + wid * (slots_per_wave + kRing) + 4 * s4) =
+        v4i{kEmpty, kEmpty, kEmpty, kEmpty};
+  __builtin_amdgcn_wave_barrier();
+  int64_t pb = 0, pe = 0;
+  pair_range(rg, ctl, pb, pe);
+  const int64_t W = (int64_t)gridDim.x * kWavesPerBlock;
+  bool bad = false;
+  for (int64_t base = pb + ((int64_t)blockIdx.x * kWavesPerBlock + wid) * G; base < pe; base += W * G) {
+    const int64_t idx = base + grp;
+    const bool has = idx < pe;
+    int32_t i = 0, ns = 0, ng = 0;
+    const int32_t* S = nullptr;
+    const int32_t* Gp = nullptr;
+    if (has) {
+      i = order_all[idx];
+      src.get(i, S, ns, Gp, ng);
+      if (ns > ng) {
+        const int32_t* t = S;
+        S = Gp;
+        Gp = t;
+        const int32_t tn = ns;
+        ns = ng;
+        ng = tn;
+      }
+    }
+    const uint32_t L = has && ns > 0 ? (uint32_t)calc_length(ns, alpha, B) : 8u;
+    bool moved = false;
+    for (int32_t j = sl; j < ns; j += LANES) {
+      const int32_t pos = lds_insert<B>(tab, L, S[j], moved);
+      bad |= pos < 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const bool displaced = __any(moved);
+    uint32_t hits = 0;
+    // G from its 16-byte aligned origin (at most 3 elements before G[0])
+    const int32_t g0 = ns > 0 && Gp ? block_origin(Gp) : 0;
+    const int32_t steps = ns > 0 ? (ng - g0 + 4 * LANES - 1) / (4 * LANES) : 0;
+    for (int32_t k = 0; __any(k < steps); ++k) {
+      int32_t x[4] = {kEmpty, kEmpty, kEmpty, kEmpty};
+      bool act[4] = {false, false, false, false};
+      const int32_t e = g0 + 4 * (k * LANES + sl);
+      if (k < steps && e < ng) {
+        const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(Gp + e));
+        x[0] = v.x;
+        x[1] = v.y;
+        x[2] = v.z;
+        x[3] = v.w;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) act[c] = e + c >= 0 && e + c < ng;
+      }
+      hits += lds_lookup<B, 4>(tab, L, displaced, x, act, bad);
+    }
+#pragma unroll
+    for (int d = LANES / 2; d > 0; d >>= 1) hits += __shfl_xor(hits, d, kWave);
+    if (has && sl == 0) sizes[i] = (int32_t)hits;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t s4 = (uint32_t)sl; s4 < L * (uint32_t)B / 4u; s4 += LANES)
+      *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (__any(bad) && lane == 0) atomicOr(&ctl[kCtlStatus], kErrTableFull);
+}
+
 // ---------------------------------------------------------------- plain form
 // intersection.cu:128-152 without the replication: every thread inserts one
 // element of the build set into the global workspace table (BN = L).
@@ -707,6 +822,19 @@ struct rs_pairs_plan {
   }
 };
 
+// A bin may run on part of a wave (lanes < 64) if the table of each of its
+// pairs fits the lane group's share of the LDS region: totals below
+// 2^(bin+1) (every bin but the last), so the smaller set has fewer than
+// 2^bin elements and is built whole.
+static bool subwave_fits(const rs_pairs_opts* o, int bin, const rs_pairs_bin& bn) {
+  if (bin >= o->num_bins - 1 || bin >= 20) return false;
+  const int32_t smax = (1 << bin) - 1;
+  const int32_t groups = kWave / bn.lanes;
+  const int32_t L = calc_length(smax > 0 ? smax : 1, bn.load_factor, bn.bucket_size);
+  return L > 0 && o->lds_slots_per_wave % (4 * groups) == 0 &&
+         (int64_t)L * bn.bucket_size <= o->lds_slots_per_wave / groups;
+}
+
 static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   RS_REQUIRE(o->num_bins >= 1 && o->num_bins <= RS_PAIRS_MAX_BINS, "pairs: bad num_bins %d",
              o->num_bins);
@@ -733,8 +861,12 @@ static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
                kMaxChunkPerLane * kWave);
     RS_REQUIRE(bn.unroll == o->bins[0].unroll && (bn.unroll == 1 || bn.unroll == 2 || bn.unroll == 4),
                "pairs: unroll must be 1, 2 or 4 and the same in every bin (bin %d)", i);
-    RS_REQUIRE(bn.bucket_size == o->bins[0].bucket_size,
-               "pairs: bucket_size must be the same in every bin (bin %d)", i);
+    RS_REQUIRE(bn.lanes == 16 || bn.lanes == 32 || bn.lanes == 64,
+               "pairs: bin %d: lanes per pair must be 16, 32 or 64 (not %d)", i, bn.lanes);
+    RS_REQUIRE(bn.lanes == 64 || subwave_fits(o, i, bn),
+               "pairs: bin %d cannot run %d lanes per pair: it must not be the last bin and a table of "
+               "2^bin - 1 keys must fit 1/%d of a wave's LDS region",
+               i, bn.lanes, kWave / bn.lanes);
     const int32_t L = calc_length(bn.chunk, bn.load_factor, bn.bucket_size);
     RS_REQUIRE(L > 0 && (int64_t)L * bn.bucket_size <= o->lds_slots_per_wave,
                "pairs: bin %d table (%d buckets x %d slots) exceeds the %d-slot LDS region", i, L,
@@ -744,39 +876,87 @@ static rs_status validate_opts(const rs_pairs_opts* o, PairsConst* cfg) {
   return RS_OK;
 }
 
+// A run of neighbouring bins launched as one kernel: the same bucket size and
+// lanes per pair (and, below 64 lanes, the same load factor, a kernel
+// argument there).  Bins are launched largest first, the order's order.
+struct PairGroup {
+  int bin_lo, bin_hi, b, lanes;
+  float alpha;
+  int max_chunk;
+};
+
+static std::vector<PairGroup> pair_groups(const rs_pairs_opts& o, int only_bin) {
+  std::vector<PairGroup> gs;
+  for (int bin = o.num_bins - 1; bin >= 0; --bin) {
+    if (only_bin >= 0 && bin != only_bin) continue;
+    const rs_pairs_bin& bn = o.bins[bin];
+    if (!gs.empty()) {
+      PairGroup& g = gs.back();
+      if (g.b == bn.bucket_size && g.lanes == bn.lanes && (bn.lanes == 64 || g.alpha == bn.load_factor) &&
+          g.bin_lo == bin + 1) {
+        g.bin_lo = bin;
+        g.max_chunk = bn.chunk > g.max_chunk ? bn.chunk : g.max_chunk;
+        continue;
+      }
+    }
+    gs.push_back(PairGroup{bin, bin, bn.bucket_size, bn.lanes, bn.load_factor, bn.chunk});
+  }
+  return gs;
+}
+
 template <int B, int U, class Src>
-static void launch_bu(int cpl, int blocks, size_t lds, const Src& src, int64_t P,
+static void launch_bu(int cpl, int blocks, size_t lds, const Src& src, const PairRange& rg,
                       const int32_t* order, int32_t* ctl, int32_t* sizes, const PairsConst& cfg,
                       int32_t spw, hipStream_t st) {
   if (cpl <= 4)
-    pairs_batched_kernel<B, U, 4, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 4, Src><<<blocks, kPairBlock, lds, st>>>(src, rg, order, ctl, sizes, cfg, spw);
   else if (cpl <= 8)
-    pairs_batched_kernel<B, U, 8, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 8, Src><<<blocks, kPairBlock, lds, st>>>(src, rg, order, ctl, sizes, cfg, spw);
   else
-    pairs_batched_kernel<B, U, 16, Src><<<blocks, kPairBlock, lds, st>>>(src, P, order, ctl, sizes, cfg, spw);
+    pairs_batched_kernel<B, U, 16, Src><<<blocks, kPairBlock, lds, st>>>(src, rg, order, ctl, sizes, cfg, spw);
 }
 
-// Launch-level shape: bucket size B and unroll U are the same in every bin;
-// CPL (build-side registers per lane) covers the largest chunk of any bin.
+template <int B, class Src>
+static void launch_sub(int lanes, int blocks, size_t lds, const Src& src, const PairRange& rg,
+                       const int32_t* order, int32_t* ctl, int32_t* sizes, float alpha, int32_t spw,
+                       hipStream_t st) {
+  if (lanes == 16)
+    pairs_subwave_kernel<B, 16, Src><<<blocks, kPairBlock, lds, st>>>(src, rg, order, ctl, sizes, alpha, spw);
+  else
+    pairs_subwave_kernel<B, 32, Src><<<blocks, kPairBlock, lds, st>>>(src, rg, order, ctl, sizes, alpha, spw);
+}
+
+// One kernel per group of bins.  from_plan: each group's positions come from
+// the plan's bin cursors; otherwise the single group covers [0, count).
 template <class Src>
-static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const Src& src, int64_t P,
-                           const int32_t* order, int32_t* ctl, int32_t* sizes, hipStream_t st) {
+static void launch_batched(const rs_pairs_opts& o, const PairsConst& cfg, const Src& src, int64_t count,
+                           bool from_plan, int only_bin, const int32_t* order, int32_t* ctl, int32_t* sizes,
+                           hipStream_t st) {
   const int blocks = device_cu_count() * o.waves_per_cu / kWavesPerBlock;
-  int max_chunk = 1;
-  for (int i = 0; i < o.num_bins; ++i) max_chunk = o.bins[i].chunk > max_chunk ? o.bins[i].chunk : max_chunk;
-  const int cpl = (max_chunk + kWave - 1) / kWave;
   const int32_t spw = o.lds_slots_per_wave;
   const size_t lds = sizeof(int32_t) * (size_t)(spw + kRing) * kWavesPerBlock;
-  const int B = o.bins[0].bucket_size, U = o.bins[0].unroll;
-#define RS_LAUNCH_BU(b_, u_) launch_bu<b_, u_, Src>(cpl, blocks, lds, src, P, order, ctl, sizes, cfg, spw, st)
-  if (B == 4) {
-    if (U == 1) RS_LAUNCH_BU(4, 1); else if (U == 2) RS_LAUNCH_BU(4, 2); else RS_LAUNCH_BU(4, 4);
-  } else if (B == 2) {
-    if (U == 1) RS_LAUNCH_BU(2, 1); else if (U == 2) RS_LAUNCH_BU(2, 2); else RS_LAUNCH_BU(2, 4);
-  } else {
-    if (U == 1) RS_LAUNCH_BU(1, 1); else if (U == 2) RS_LAUNCH_BU(1, 2); else RS_LAUNCH_BU(1, 4);
-  }
+  const int U = o.bins[0].unroll;
+  const std::vector<PairGroup> gs = pair_groups(o, only_bin);
+  for (size_t gi = 0; gi < gs.size(); ++gi) {
+    const PairGroup& g = gs[gi];
+    const PairRange rg{0, count, from_plan ? g.bin_lo : -1, from_plan ? g.bin_hi : -1, kCtlNext + (int)gi};
+    if (g.lanes < 64) {
+      if (g.b == 4) launch_sub<4, Src>(g.lanes, blocks, lds, src, rg, order, ctl, sizes, g.alpha, spw, st);
+      else if (g.b == 2) launch_sub<2, Src>(g.lanes, blocks, lds, src, rg, order, ctl, sizes, g.alpha, spw, st);
+      else launch_sub<1, Src>(g.lanes, blocks, lds, src, rg, order, ctl, sizes, g.alpha, spw, st);
+      continue;
+    }
+    const int cpl = (g.max_chunk + kWave - 1) / kWave;
+#define RS_LAUNCH_BU(b_, u_) launch_bu<b_, u_, Src>(cpl, blocks, lds, src, rg, order, ctl, sizes, cfg, spw, st)
+    if (g.b == 4) {
+      if (U == 1) RS_LAUNCH_BU(4, 1); else if (U == 2) RS_LAUNCH_BU(4, 2); else RS_LAUNCH_BU(4, 4);
+    } else if (g.b == 2) {
+      if (U == 1) RS_LAUNCH_BU(2, 1); else if (U == 2) RS_LAUNCH_BU(2, 2); else RS_LAUNCH_BU(2, 4);
+    } else {
+      if (U == 1) RS_LAUNCH_BU(1, 1); else if (U == 2) RS_LAUNCH_BU(1, 2); else RS_LAUNCH_BU(1, 4);
+    }
 #undef RS_LAUNCH_BU
+  }
 }
 
 template <class Src>
@@ -792,7 +972,7 @@ static rs_status pairs_enqueue(rs_pairs_plan* plan, const Src& src, int64_t P, i
     plan_bins_scan<<<1, 64, 0, st>>>(nb, ctl);
     plan_scatter<Src><<<g, kPlanBlock, 0, st>>>(src, P, nb, ctl, order);
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev0, st));
-    launch_batched<Src>(plan->opts, plan->cfg, src, P, order, ctl, d_sizes, st);
+    launch_batched<Src>(plan->opts, plan->cfg, src, P, true, -1, order, ctl, d_sizes, st);
     if (plan->timing) RS_HIP(hipEventRecord(plan->ev1, st));
   }
   RS_HIP(hipGetLastError());
@@ -806,19 +986,23 @@ static rs_status pairs_enqueue(rs_pairs_plan* plan, const int64_t* d_a_off, cons
 }
 
 namespace rs {
-// Runs the batched kernel over the pairs listed in d_order[0..count) only
-// (used by the autotuner to time one size bin at a time).
-rs_status pairs_launch_ordered(const rs_pairs_opts* opts, const int64_t* d_a_off,
+// Runs the kernel of bin `bin`'s settings over the pairs listed in
+// d_order[0..count) only (the autotuner times one size bin at a time).
+rs_status pairs_launch_ordered(const rs_pairs_opts* opts, int bin, const int64_t* d_a_off,
                                const int32_t* d_a, const int64_t* d_b_off, const int32_t* d_b,
                                const int32_t* d_order, int64_t count, int32_t* d_ctl,
                                int32_t* d_sizes, hipStream_t st) {
   PairsConst cfg{};
   RS_TRY(validate_opts(opts, &cfg));
+  RS_REQUIRE(bin >= 0 && bin < opts->num_bins, "pairs: bad bin %d", bin);
   plan_reset<<<1, 64, 0, st>>>(d_ctl);
-  launch_batched<PairSrcCsr>(*opts, cfg, PairSrcCsr{d_a_off, d_a, d_b_off, d_b}, count, d_order, d_ctl,
-                             d_sizes, st);
+  launch_batched<PairSrcCsr>(*opts, cfg, PairSrcCsr{d_a_off, d_a, d_b_off, d_b}, count, false, bin, d_order,
+                             d_ctl, d_sizes, st);
   RS_HIP(hipGetLastError());
   return RS_OK;
+}
+bool pairs_subwave_fits(const rs_pairs_opts* opts, int bin, const rs_pairs_bin* b) {
+  return subwave_fits(opts, bin, *b);
 }
 int pairs_ctl_words() { return kCtlWords; }
 int pairs_ctl_status_index() { return kCtlStatus; }
@@ -842,7 +1026,7 @@ void rs_pairs_default_opts(rs_pairs_opts* o) {
     // the 2048-slot region at <= 1/2 key per 4-slot bucket, where a home
     // bucket overflows (forcing the walking lookup) in < 10% of chunks; two
     // blocks per lane in flight where one set is >= 4x the other.
-    o->bins[i] = rs_pairs_bin{0.125f, 4, 255, 2};
+    o->bins[i] = rs_pairs_bin{0.125f, 4, 255, 2, 64};
   }
 }
 

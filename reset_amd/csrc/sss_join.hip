@@ -641,7 +641,7 @@ static rs_status sizes_by_groups(const int32_t* a_off, const int32_t* a_el, cons
     }
     RS_REQUIRE(chunk > 0, "sss: no 64-element chunk fits a pair table at load_factor %g", (double)alpha);
     if (alpha != kVerifyAlpha || bsz != kVerifyB)
-      for (int i = 0; i < o.num_bins; ++i) o.bins[i] = rs_pairs_bin{alpha, bsz, chunk, o.bins[i].unroll};
+      for (int i = 0; i < o.num_bins; ++i) o.bins[i] = rs_pairs_bin{alpha, bsz, chunk, o.bins[i].unroll, 64};
     rs_pairs_plan* plan = nullptr;
     RS_TRY(rs_pairs_plan_create(R, &o, &plan));
     struct PGuard {

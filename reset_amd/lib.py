@@ -39,7 +39,7 @@ class rs_tables_desc(C.Structure):
 
 class rs_pairs_bin(C.Structure):
     _fields_ = [("load_factor", C.c_float), ("bucket_size", C.c_int32), ("chunk", C.c_int32),
-                ("unroll", C.c_int32)]
+                ("unroll", C.c_int32), ("lanes", C.c_int32)]
 
 
 class rs_pairs_opts(C.Structure):

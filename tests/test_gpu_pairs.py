@@ -268,7 +268,8 @@ def test_autotuned_opts_stay_bit_exact(api, oracle_lib):
     batch = inputs.make_pair_batch(3000, seed=77, lo=16, hi=20000)
     o = api.autotune_pairs(batch["a_off"], batch["a"], batch["b_off"], batch["b"], reps=1)
     table = api.pairs_opts_table(o)
-    assert all(b == 4 and 1 <= c <= 1024 and 0 < al <= 1 for al, b, c, _ in table)
+    assert all(b in (1, 2, 4) and 64 <= c <= 1024 and 0 < al <= 1 and lanes in (16, 32, 64)
+               for al, b, c, _, lanes in table)
     plan = api.PairsPlan(3000, o)
     sizes = torch.empty(3000, dtype=torch.int32, device="cuda")
     plan.run_async(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes)
@@ -329,3 +330,68 @@ def test_batched_flags_unsorted_sets_on_the_chunked_path(api, oracle_lib):
     got = plan.run(*_dev(ao, np.concatenate(A), bo, np.concatenate(B))).cpu().numpy()
     assert np.array_equal(got, k[idx])
     plan.close()
+
+
+def _mixed_opts(api):
+    """Bins with different bucket sizes and lanes per pair: bins 0-3 on 16
+    lanes (b 1), 4-5 on 32 lanes (b 2), 6 on 16 lanes (b 4), 7-9 a wave with
+    b 2, 10-12 a wave with b 1, the rest the defaults (b 4) -- six launches."""
+    opts = api.default_pairs_opts()
+    def put(bins, alpha, b, chunk, lanes):
+        for i in bins:
+            opts.bins[i].load_factor, opts.bins[i].bucket_size = alpha, b
+            opts.bins[i].chunk, opts.bins[i].lanes = chunk, lanes
+    put(range(0, 4), 0.125, 1, 255, 16)
+    put(range(4, 6), 0.25, 2, 255, 32)
+    put([6], 0.125, 4, 255, 16)
+    put(range(7, 10), 0.125, 2, 127, 64)
+    put(range(10, 13), 0.0625, 1, 127, 64)
+    return opts
+
+
+def test_mixed_bin_settings_stay_bit_exact(api, oracle_lib):
+    """Per-bin bucket size and probe width (lanes per pair), launched as one
+    kernel per run of equal settings: sizes equal the constructed overlaps and
+    the oracle, eagerly and from graph replays."""
+    from reset_amd import inputs
+
+    batch = inputs.make_pair_batch(6000, seed=19, lo=1, hi=40000)
+    opts = _mixed_opts(api)
+    plan = api.PairsPlan(6000, opts)
+    got = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"])
+    assert torch.equal(got, batch["k"])
+    ha, hb = batch["a_off"].cpu().numpy(), batch["b_off"].cpu().numpy()
+    ref = oracle_lib.pairs_merge(ha, batch["a"].cpu().numpy(), hb, batch["b"].cpu().numpy())
+    assert np.array_equal(ref, got.cpu().numpy())
+    # bins 0..6 hold pairs (sizes from 1), so the part-wave kernels ran
+    tot = np.diff(ha) + np.diff(hb)
+    assert (tot < 128).sum() > 50
+    s = torch.cuda.Stream()
+    sizes = torch.zeros(6000, dtype=torch.int32, device="cuda")
+    with torch.cuda.stream(s):
+        plan.capture(batch["a_off"], batch["a"], batch["b_off"], batch["b"], sizes, s)
+    for rep in range(3):
+        sizes.fill_(-7)
+        torch.cuda.synchronize()
+        plan.replay(s)
+        s.synchronize()
+        assert torch.equal(sizes, batch["k"]), rep
+        torch.cuda.synchronize()
+    plan.status()
+    plan.close()
+
+
+def test_part_wave_lanes_rejected_where_they_do_not_fit(api):
+    """16 lanes per pair in the last bin, or where a table of 2^bin - 1 keys
+    exceeds a quarter of the wave's region, is refused at plan creation."""
+    from reset_amd.lib import ResetError
+
+    for bin_, lanes in ((19, 16), (12, 16), (9, 32)):
+        opts = api.default_pairs_opts()
+        opts.bins[bin_].lanes = lanes
+        with pytest.raises(ResetError):
+            api.PairsPlan(10, opts)
+    opts = api.default_pairs_opts()
+    opts.bins[3].lanes = 17
+    with pytest.raises(ResetError):
+        api.PairsPlan(10, opts)
