@@ -308,6 +308,16 @@ __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, b
     lds_read_bucket<B>(tab, j[n], v[n]);
   }
   uint32_t hits = 0;
+  if (!displaced) {  // (wave-uniform) every key sits in its home bucket
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
+#pragma unroll
+      for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
+      hits += (uint32_t)(act[n] && m == 0u);
+    }
+    return hits;
+  }
   bool pend[N];
   bool any = false;
 #pragma unroll
@@ -320,7 +330,7 @@ __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, b
     pend[n] = act[n] && !hit && v[n][B - 1] != kEmpty;
     any |= pend[n];
   }
-  if (displaced && __any(any)) {  // (uniform test first: pend is dead code otherwise)
+  if (__any(any)) {
 #pragma unroll
     for (int n = 0; n < N; ++n) {
       bool p = pend[n];
@@ -484,15 +494,20 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
   };
   bool displaced = build();
   int32_t pos = block_origin(G);
-  GBlock<U> cur;
-  load_block<U>(G, pos, 0, ng, cur);
+  // G streams through three block buffers in fixed roles, as in the
+  // resident path: each step loads the block two ahead into the buffer
+  // probed in the step before, so two blocks are in flight while one is
+  // probed.  The step's S group is loaded before its block, so that waiting
+  // for the group when it is staged at the end of the step leaves the block
+  // loads behind it in flight.
+  GBlock<U> b0, b1, b2;
+  load_block<U>(G, pos, 0, ng, b0);
+  load_block<U>(G, pos + kBlock, 0, ng, b1);
   uint32_t hits = 0;
-  while (true) {
-    GBlock<U> nxt;
-    const bool more = pos + kBlock < ng;
-    load_block<U>(G, pos + kBlock, 0, ng, nxt);  // unconditional, as above
+  auto step = [&](GBlock<U>& cur, GBlock<U>& ahead) -> bool {
     const int32_t staged_g = sc;
-    const v4i staged = group_load(staged_g);     // unconditional, as above
+    const v4i staged = group_load(staged_g);          // unconditional, as above
+    load_block<U>(G, pos + 2 * kBlock, 0, ng, ahead);  // unconditional, as above
     // an element below lo can only be met on the block's first pass: the
     // elements left for a later chunk lie above this chunk's top, which is
     // one below the next chunk's lo
@@ -518,7 +533,7 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       last = ce >= ns;
       displaced = build();
     }
-    if (!more) break;
+    if (pos + kBlock >= ng) return false;
     // stage the group loaded above if it is still the next one and its ring
     // slot is free (the group two before it lies below the current chunk)
     if (staged_g == sc && sc <= group_of(cb) + 1) {
@@ -527,7 +542,9 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
       ++sc;
     }
     pos += kBlock;
-    cur = nxt;
+    return true;
+  };
+  while (step(b0, b2) && step(b1, b0) && step(b2, b1)) {
   }
   clear_chunk<4>(tab, spos);
   __builtin_amdgcn_wave_barrier();
