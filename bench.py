@@ -24,6 +24,8 @@ of the reference algorithm (oracle/) on a bounded sample of the same pairs.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -552,6 +554,20 @@ def bench_ir_batch(args):
                          "alg_bytes_per_launch": alg, "term0_entries": n0, "probes": probes}}
 
 
+@contextlib.contextmanager
+def _no_gc():
+    """The arrival loop runs without Python's cyclic collector: a full pass
+    over the results held by then (millions of ints) pauses the submitting
+    thread for ~0.1 s, and the requests due meanwhile would arrive as one
+    burst."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def _serve_at(t, queries, qps, n, args, seed, layout):
     """Poisson arrivals at `qps` of the first n queries through the serving
     loop -> (latencies ms, span ms, results by id, stats)."""
@@ -563,14 +579,15 @@ def _serve_at(t, queries, qps, n, args, seed, layout):
     gaps = rng.exponential(1.0 / qps, n)
     srv = serving.Server(t, layout=layout, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch,
                          max_inflight=4)
-    start = time.perf_counter()
-    due = start + np.cumsum(gaps)
-    for i in range(n):
-        d = due[i] - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        srv.submit(i, queries[i])
-    srv.drain()
+    with _no_gc():
+        start = time.perf_counter()
+        due = start + np.cumsum(gaps)
+        for i in range(n):
+            d = due[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(i, queries[i])
+        srv.drain()
     comps = srv.poll_all()
     lat = np.array([c.done_ms - c.submit_ms for c in comps])
     span_ms = max(c.done_ms for c in comps) - min(c.submit_ms for c in comps)
@@ -647,7 +664,7 @@ def bench_ir(args):
         "batch_all_queries": batch,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ir_segment_kernel + ir_gather_kernel (all queries in one batch)",
+                     "kernel": "ir_segment_kernel with its count, scan and copy kernels (all queries in one batch)",
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes,
                      "term0_entries": n0, "probes": probes},
         "parity": {"served_equals_batch": bool(served_ok)},
@@ -724,22 +741,23 @@ def _mixed_serving(args, t, queries, resh, cnth, roffh):
     rate = args.qps + args.sss_qps
     gaps = rng.exponential(1.0 / rate, n_ir + n_s)
     kinds = rng.permutation(np.r_[np.zeros(n_ir, np.int8), np.ones(n_s, np.int8)])
-    srv = serving.Server(t, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch, max_inflight=4,
-                         sss_index=sidx)
-    start = time.perf_counter()
-    due = start + np.cumsum(gaps)
-    ii = si = 0
-    for k in range(n_ir + n_s):
-        d = due[k] - time.perf_counter()
-        if d > 0:
-            time.sleep(d)
-        if kinds[k] == 0:
-            srv.submit_ir(ii, queries[ii])
-            ii += 1
-        else:
-            srv.submit_sss(1 << 40 | si, lookups[si])
-            si += 1
-    srv.drain()
+    srv = serving.Server(t, layout=1, budget_bytes=args.ir_budget_mb << 20, max_batch=args.ir_batch,
+                         max_inflight=args.mixed_inflight, sss_index=sidx)
+    with _no_gc():
+        start = time.perf_counter()
+        due = start + np.cumsum(gaps)
+        ii = si = 0
+        for k in range(n_ir + n_s):
+            d = due[k] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            if kinds[k] == 0:
+                srv.submit_ir(ii, queries[ii])
+                ii += 1
+            else:
+                srv.submit_sss(1 << 40 | si, lookups[si])
+                si += 1
+        srv.drain()
     comps = srv.poll_all()
     lat = {0: [], 1: []}
     ok_ir = ok_s = True
@@ -755,7 +773,7 @@ def _mixed_serving(args, t, queries, resh, cnth, roffh):
     srv.close()
     sidx.close()
     pct = lambda v, p: float(np.percentile(np.array(v), p)) if v else None
-    return {"ir_queries": n_ir, "sss_lookups": n_s, "arrival_qps": rate,
+    return {"ir_queries": n_ir, "sss_lookups": n_s, "arrival_qps": rate, "ir_streams": args.mixed_inflight,
             "sss_collection": {"sets": int(len(soff_h) - 1), "threshold": "4/5"},
             "ir_latency_ms": {"p50": pct(lat[0], 50), "p99": pct(lat[0], 99)},
             "sss_latency_ms": {"p50": pct(lat[1], 50), "p99": pct(lat[1], 99)},
@@ -982,6 +1000,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="tc/ir: CPU baseline sample grown to this")
     ap.add_argument("--ir-budget-mb", type=int, default=4096)
     ap.add_argument("--ir-batch", type=int, default=256)
+    ap.add_argument("--mixed-inflight", type=int, default=4,
+                    help="IR streams of the mixed serving run (the lookups take one more)")
     ap.add_argument("--sss-plant", type=int, default=1000,
                     help="sss workload: near-duplicate sets planted at the end of the collection")
     ap.add_argument("--sss-lookups", type=int, default=0,

@@ -294,7 +294,9 @@ rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_bu
 rs_status rs_server_submit_ir(rs_server* s, int64_t query_id, const int32_t* h_terms,
                               int32_t num_terms);
 /* Attach a similarity index (kept borrowed; it must outlive the server) and
- * start the lookup thread.  Once per server. */
+ * start the lookup thread.  Once per server.  From then on IR batches use at
+ * most GPU_MAX_HW_QUEUES - 1 (3 by default) of the max_inflight slots, so the
+ * lookup stream keeps a hardware queue of its own. */
 rs_status rs_server_attach_sss(rs_server* s, const rs_sss_index* idx);
 /* Enqueue one similarity lookup: the query set's tokens on the host,
  * strictly ascending and non-negative.  Its result is the ascending list of

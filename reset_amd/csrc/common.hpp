@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/reset_amd.h"
 
@@ -140,10 +141,47 @@ inline int grid_for(int64_t items, int block, int max_blocks_per_cu = 8) {
   return g < 1 ? 1 : (int)g;
 }
 
+// Cache of device blocks for one host thread.  While a thread has a cache
+// installed (BlockCacheScope), DevBuf takes blocks from it and gives them back
+// to it instead of calling hipMalloc/hipFree: hipFree waits for the whole
+// device, so a serving thread that allocates per request would otherwise stall
+// every other stream's queued work at each release.  Blocks are sized in
+// powers of two (at least 4 KiB) and reused only by the thread that installed
+// the cache, whose work is ordered on one stream: a block given back while
+// kernels that read it are still queued is handed out again only to work
+// queued behind them on that stream.  Free blocks beyond `max_free_bytes` are
+// released with hipFree.
+struct BlockCache {
+  explicit BlockCache(size_t max_free_bytes) : cap(max_free_bytes) {}
+  BlockCache(const BlockCache&) = delete;
+  BlockCache& operator=(const BlockCache&) = delete;
+  ~BlockCache() {
+    for (int c = 0; c < kClasses; ++c)
+      for (void* q : free_blocks[c]) (void)hipFree(q);
+  }
+  static constexpr int kClasses = 48;
+  static int size_class(size_t n) {
+    int c = 12;
+    while (c < kClasses - 1 && ((size_t)1 << c) < n) ++c;
+    return c;
+  }
+  size_t cap;
+  size_t held = 0;  // bytes in free_blocks
+  std::vector<void*> free_blocks[kClasses];
+};
+inline thread_local BlockCache* tl_block_cache = nullptr;
+
+struct BlockCacheScope {
+  BlockCache* prev;
+  explicit BlockCacheScope(BlockCache* c) : prev(tl_block_cache) { tl_block_cache = c; }
+  ~BlockCacheScope() { tl_block_cache = prev; }
+};
+
 // Device buffer owned by host objects.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  BlockCache* owner = nullptr;  // the cache the block returns to, if any
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -151,6 +189,23 @@ struct DevBuf {
   rs_status alloc(size_t n) {
     release();
     if (n == 0) n = 16;
+    BlockCache* c = tl_block_cache;
+    if (c) {
+      const int k = BlockCache::size_class(n);
+      if ((size_t)1 << k >= n) {
+        n = (size_t)1 << k;
+        if (!c->free_blocks[k].empty()) {
+          p = c->free_blocks[k].back();
+          c->free_blocks[k].pop_back();
+          c->held -= n;
+          bytes = n;
+          owner = c;
+          return RS_OK;
+        }
+      } else {
+        c = nullptr;  // beyond the largest class: a plain allocation
+      }
+    }
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -158,12 +213,19 @@ struct DevBuf {
       return RS_ENOMEM;
     }
     bytes = n;
+    owner = c;
     return RS_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owner && owner->held + bytes <= owner->cap) {
+      owner->free_blocks[BlockCache::size_class(bytes)].push_back(p);
+      owner->held += bytes;
+    } else if (p) {
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    owner = nullptr;
   }
   template <typename T>
   T* as() const { return reinterpret_cast<T*>(p); }

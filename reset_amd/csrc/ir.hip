@@ -10,7 +10,9 @@
 // prefix scan of the segments' survivor counts then places every segment's
 // survivors at once: in each query's result region (ir.cu:324-330 sizing) or
 // densely, query after query, for the server.
+#include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <memory>
@@ -508,6 +510,9 @@ struct rs_server {
   std::deque<IrPending> sss_pending;  // terms = the query set's tokens
   std::thread sss_worker;
   hipStream_t sss_stream = nullptr;
+  // IR slots in use while lookups are attached: one hardware queue fewer
+  // than HIP gives the process, so the lookup stream keeps a queue of its own
+  int32_t ir_slots_with_sss = 0;
   int32_t* h_sss_tok = nullptr;   // pinned staging
   int32_t* h_sss_off = nullptr;
   int64_t* h_sss_keys = nullptr;
@@ -643,8 +648,9 @@ static void server_loop(rs_server* S) {
       std::unique_lock<std::mutex> lk(S->mu);
       if (S->stop) break;
       if (!S->paused) expire_locked(S, S->pending, RS_REQ_IR);
-      for (auto& sp : S->slots) {
-        IrSlot& s = *sp;
+      const size_t active = S->sss ? (size_t)S->ir_slots_with_sss : S->slots.size();
+      for (size_t si = 0; si < active; ++si) {
+        IrSlot& s = *S->slots[si];
         if (S->paused || s.state != 0 || S->pending.empty()) continue;
         int64_t bytes = 0;
         while (!S->pending.empty() && (int32_t)s.batch.size() < S->max_batch) {
@@ -759,6 +765,10 @@ static rs_status sss_run_batch(rs_server* S, std::vector<IrPending>& batch) {
 
 static void sss_loop(rs_server* S) {
   (void)hipSetDevice(S->device);
+  // the lookups' device buffers come from a cache kept for this thread: a
+  // hipFree per batch would wait for the whole device, IR batches included
+  BlockCache blocks((size_t)S->budget);
+  BlockCacheScope cached(&blocks);
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
     S->cv.wait(lk, [S] { return S->stop || (!S->paused && !S->sss_pending.empty()); });
@@ -880,9 +890,26 @@ rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_bu
   }
   S->slot_budget = memory_budget_bytes / max_inflight;
   const int64_t cap_entries = S->slot_budget / 8;
+  // HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless the
+  // environment says otherwise); two streams on one queue serialize.  The
+  // lookup stream is created right after the slots that stay in use when
+  // lookups are attached, so it does not share their queues.
+  int32_t hw_queues = 4;
+  if (const char* env = std::getenv("GPU_MAX_HW_QUEUES")) {
+    const int v = std::atoi(env);
+    if (v > 0) hw_queues = v;
+  }
+  S->ir_slots_with_sss = std::max(1, std::min(max_inflight, hw_queues - 1));
   for (int i = 0; i < max_inflight; ++i) S->slots.push_back(std::make_unique<IrSlot>());
-  for (auto& sp : S->slots) {
-    IrSlot& s = *sp;
+  for (int i = 0; i < max_inflight; ++i) {
+    IrSlot& s = *S->slots[(size_t)i];
+    if (i == S->ir_slots_with_sss &&
+        hipStreamCreateWithFlags(&S->sss_stream, hipStreamNonBlocking) != hipSuccess) {
+      S->sss_stream = nullptr;
+      rs_server_free(S);
+      set_error("rs_server_create: stream creation failed");
+      return RS_EDEVICE;
+    }
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s.ev_counts, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s.ev_done, hipEventDisableTiming) != hipSuccess) {
@@ -897,6 +924,12 @@ rs_status rs_server_create(const rs_tables* t, int32_t layout, int64_t memory_bu
       set_error("rs_server_create: %s", msg.c_str());
       return st;
     }
+  }
+  if (!S->sss_stream && hipStreamCreateWithFlags(&S->sss_stream, hipStreamNonBlocking) != hipSuccess) {
+    S->sss_stream = nullptr;
+    rs_server_free(S);
+    set_error("rs_server_create: stream creation failed");
+    return RS_EDEVICE;
   }
   S->worker = std::thread(server_loop, S);
   *out = S;
@@ -937,16 +970,6 @@ rs_status rs_server_attach_sss(rs_server* S, const rs_sss_index* idx) {
   RS_REQUIRE(S && idx, "rs_server_attach_sss: null argument");
   std::lock_guard<std::mutex> lk(S->mu);
   RS_REQUIRE(!S->sss, "rs_server_attach_sss: an index is already attached");
-  int cur = 0;
-  (void)hipGetDevice(&cur);
-  (void)hipSetDevice(S->device);
-  const hipError_t e = hipStreamCreateWithFlags(&S->sss_stream, hipStreamNonBlocking);
-  (void)hipSetDevice(cur);
-  if (e != hipSuccess) {
-    S->sss_stream = nullptr;
-    set_error("rs_server_attach_sss: stream creation failed");
-    return RS_EDEVICE;
-  }
   S->sss = idx;
   S->sss_worker = std::thread(sss_loop, S);
   return RS_OK;

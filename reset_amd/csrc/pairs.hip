@@ -1228,7 +1228,10 @@ rs_status rs_pairs_intersect_plain(const int64_t* d_a_off, const int32_t* d_a,
   RS_HIP(hipGetDevice(&dev));
   RS_REQUIRE(dev >= 0 && dev < 64, "plain pairs: device ordinal %d out of range", dev);
   static thread_local DevBuf err_buf[64];
-  if (!err_buf[dev].p) RS_TRY(err_buf[dev].alloc(16));
+  if (!err_buf[dev].p) {
+    BlockCacheScope uncached(nullptr);  // the word lives as long as the thread
+    RS_TRY(err_buf[dev].alloc(16));
+  }
   int32_t* err = err_buf[dev].as<int32_t>();
   set_word_kernel<<<1, 1, 0, st>>>(err, 0);
   for (int64_t i = 0; i < num_pairs; ++i) {

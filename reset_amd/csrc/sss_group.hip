@@ -7,7 +7,9 @@
 // B-slot buckets with B in {1, 2, 4}, load factor alpha: the table's
 // hyperparameters, 1/4 and 4 by default), then its waves stream the
 // partner sets S_v through it, one candidate per wave at a time.  Per pair
-// the cost is the streamed set alone; the build is paid once per unit.
+// the cost is the streamed set alone; the build is paid once per unit.  A
+// short candidate list (a few lookups) is cut into shorter units, down to 8
+// candidates, so that it still spreads over the whole device.
 //
 // Probe rule (the same walk as TC.cu:11-31, a bucket at a time): inserts take
 // the first EMPTY slot of their walk and never clear one, so a bucket's
@@ -51,14 +53,16 @@ __device__ __forceinline__ void group_bucket(const int32_t* __restrict__ tab, ui
   }
 }
 
+constexpr int64_t kGroupMinCands = 8;
+
 // unit_start[k] = first candidate of unit k: a candidate opens a unit when
 // its u differs from the previous one's or its index is a multiple of
-// kGroupCands.
-__global__ void group_flags_kernel(const int32_t* __restrict__ cu, int64_t n,
+// unit_len (<= kGroupCands).
+__global__ void group_flags_kernel(const int32_t* __restrict__ cu, int64_t n, int64_t unit_len,
                                    int32_t* __restrict__ flags) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    flags[i] = i == 0 || cu[i] != cu[i - 1] || i % kGroupCands == 0;
+    flags[i] = i == 0 || cu[i] != cu[i - 1] || i % unit_len == 0;
 }
 
 // Streaming the partner sets: each wave takes the unit's candidates
@@ -296,7 +300,12 @@ rs_status group_verify(float alpha, int32_t b, const int32_t* a_off, const int32
   RS_TRY(flags.alloc(sizeof(int32_t) * (size_t)n));
   RS_TRY(starts.alloc(sizeof(int64_t) * (size_t)n));
   RS_TRY(nsel.alloc(16));
-  group_flags_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(cu, n, flags.as<int32_t>());
+  const int grid = device_cu_count() * 4;  // 32 KiB of LDS each: 4 workgroups per CU
+  // units of at most kGroupCands candidates, shorter when that leaves fewer
+  // than two units per workgroup
+  int64_t unit_len = (n + 2 * grid - 1) / (2 * grid);
+  unit_len = unit_len < kGroupMinCands ? kGroupMinCands : unit_len > kGroupCands ? kGroupCands : unit_len;
+  group_flags_kernel<<<grid_for(n, 256, 8), 256, 0, st>>>(cu, n, unit_len, flags.as<int32_t>());
   rocprim::counting_iterator<int64_t> idx(0);
   size_t bytes = 0;
   RS_ROC_G(rocprim::select(nullptr, bytes, idx, flags.as<int32_t>(), starts.as<int64_t>(),
@@ -307,7 +316,6 @@ rs_status group_verify(float alpha, int32_t b, const int32_t* a_off, const int32
   size_t nunits = 0;
   RS_HIP(hipMemcpyAsync(&nunits, nsel.p, sizeof(size_t), hipMemcpyDeviceToHost, st));
   RS_HIP(hipStreamSynchronize(st));
-  const int grid = device_cu_count() * 4;  // 32 KiB of LDS each: 4 workgroups per CU
   // the work-stealing counter: the second word of nsel (the first holds the
   // select count read above)
   auto* next_unit = reinterpret_cast<unsigned long long*>(nsel.as<size_t>() + 1);
