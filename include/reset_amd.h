@@ -164,7 +164,9 @@ rs_status rs_tc_count_async(const rs_tables* t, const rs_tc_plan* p, int32_t lay
 rs_status rs_tc_count_plain_async(const rs_tables* t, const rs_tc_plan* p, int32_t layout,
                                   const int32_t* d_probe_keys, uint64_t* d_count,
                                   rs_stream stream);
-/* Convenience: plan (cached in t), count, synchronize, *h_count. */
+/* Convenience: plan (cached in t), count, synchronize, *h_count.  Calls on
+ * one t from several host threads take turns (the cached plan's counters
+ * serve one count at a time). */
 rs_status rs_tc_count(rs_tables* t, int32_t layout, const int32_t* d_probe_keys,
                       uint64_t* h_count, rs_stream stream);
 

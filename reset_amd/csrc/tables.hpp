@@ -1,6 +1,8 @@
 // tables.hpp -- the per-set hash table family (host-side handle).
 #pragma once
 
+#include <mutex>
+
 #include "common.hpp"
 
 struct rs_tc_plan;
@@ -20,6 +22,9 @@ struct rs_tables {
   rs::DevBuf status;     // int32 error bits, then 2 x uint64 conflict counters
   int64_t conflicts[2] = {0, 0};
   rs_tc_plan* tc_plan = nullptr;  // cached by rs_tc_count
+  // rs_tc_count holds it from the plan's creation to the count's end: the
+  // cached plan's work-stealing counters serve one count at a time
+  std::mutex tc_mu;
   // rs_tables_timing: events around the kernels of the last TC count / IR
   // query on this handle
   bool timing = false;

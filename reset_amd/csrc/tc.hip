@@ -1009,6 +1009,7 @@ rs_status rs_tc_count_plain_async(const rs_tables* t, const rs_tc_plan* p, int32
 rs_status rs_tc_count(rs_tables* t, int32_t layout, const int32_t* d_probe_keys,
                       uint64_t* h_count, rs_stream stream) {
   RS_REQUIRE(t && h_count, "rs_tc_count: null argument");
+  std::lock_guard<std::mutex> lk(t->tc_mu);
   if (!t->tc_plan) RS_TRY(rs_tc_plan_create(t, stream, &t->tc_plan));
   DevBuf out;
   RS_TRY(out.alloc(sizeof(uint64_t)));
