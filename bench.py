@@ -964,6 +964,11 @@ BASELINE_METRIC = "triangles/s & set-pair intersections/s (1 GPU); IR query p50/
 _HEADER = ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "dtype", "data")
 
 
+def _progress(msg):
+    """One line on stderr when a block of the default run ends."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def _block(d):
     return {k: v for k, v in d.items() if k not in _HEADER}
 
@@ -975,13 +980,18 @@ def bench_all(args):
     CPU baseline."""
     import torch
 
+    t0 = time.perf_counter()
     out = bench_pairs(args)
+    _progress(f"pairs block done ({time.perf_counter() - t0:.0f} s)")
     torch.cuda.empty_cache()
     tc24 = bench_tc(args, 24)
+    _progress(f"tc scale-24 block done ({time.perf_counter() - t0:.0f} s)")
     torch.cuda.empty_cache()
     tc16 = bench_tc(args, 16)
+    _progress(f"tc scale-16 block done ({time.perf_counter() - t0:.0f} s)")
     torch.cuda.empty_cache()
     ir = bench_ir(args)
+    _progress(f"ir block done ({time.perf_counter() - t0:.0f} s)")
     out["metric"] = BASELINE_METRIC
     out["headline"] = ("value: set-pair intersections/s of the 1M-pair batch; triangles/s of R-MAT scale 24 in "
                        "triangles_per_s (block tc); IR p50/p99 at fixed QPS in ir_latency_ms (block ir)")
