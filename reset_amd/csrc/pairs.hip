@@ -12,8 +12,11 @@
 // minimum 4(|A|+|B|) bytes).  A smaller set above the bin's chunk is built
 // chunk by chunk in value order (both sets ascending), each chunk answering
 // for the larger set's elements up to its own last element; its elements are
-// staged through LDS ahead of use.  Everything stays integer; the per-pair
-// count is reduced across the wave and written once.
+// staged through LDS ahead of use.  Where the sets are within 8x of each
+// other a chunk's table is built without atomics: its keys go in set order
+// to the slots a sequential insert would give them, found by a prefix max
+// over the wave (pair_ordered_wave).  The per-pair count is reduced across
+// the wave and written once.
 #include <vector>
 
 #include "common.hpp"
@@ -213,6 +216,11 @@ __device__ __forceinline__ void lds_read_bucket(const int32_t* tab, uint32_t buc
 
 constexpr int kMaxChunkPerLane = 16;  // chunk <= 1024 build-side elements
 constexpr int32_t kWideRatio = 4;     // |G| >= 4 |S|: stream U blocks per lane
+// Chunked pairs with |G| >= 8 |S| rebuild rarely and take the hashed chunk
+// tables (one AND per lookup); below that the ordered chunk tables, whose
+// build needs no atomics, win (1M pairs of 16-65536: 11.59 -> 11.03 ms).
+constexpr int32_t kOrderedRatio = 8;
+static_assert(kOrderedRatio >= kWideRatio, "the hashed chunk path runs wide only");
 
 // One block of the larger set: 4*U elements per lane, 256*U per wave, from a
 // 16-byte aligned position `pos` (which may lie up to 3 elements before the
@@ -289,17 +297,30 @@ __device__ __forceinline__ void load_chunk(const int32_t* __restrict__ S, int32_
   }
 }
 
+// Whether x is among the B slots of a bucket: min(slot ^ x) == 0, on the
+// vector ALU only.  (Counting the matches on the scalar unit instead, from
+// compare masks and a ballot's population count, measured slower: 11.3 ms
+// against 11.0 ms on the 1M-pair batch.)
+template <int B>
+__device__ __forceinline__ bool bucket_has(const int32_t (&v)[B], int32_t x) {
+  uint32_t m = (uint32_t)(v[0] ^ x);
+#pragma unroll
+  for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[k] ^ x));
+  return m == 0u;
+}
+
 // Lookups of N keys per lane.  Buckets fill as prefixes (an insert takes the
 // first EMPTY slot of its walk and no slot is emptied while the table lives),
 // so one bucket read decides a key unless that bucket is full and some key of
 // the table was placed outside its home bucket (`displaced`); only such keys
 // walk on, bucket by bucket, until they meet themselves or a bucket with an
-// EMPTY slot.  any(slot == x) is evaluated as min(slot ^ x) == 0.
+// EMPTY slot.  Returns the lane's hit count.
 template <int B, int N>
 __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, bool displaced,
                                                const int32_t (&x)[N], const bool (&act)[N],
                                                bool& bad) {
   const uint32_t mask = L - 1u;
+  uint32_t hits = 0;
   int32_t v[N][B];
   uint32_t j[N];
 #pragma unroll
@@ -307,25 +328,16 @@ __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, b
     j[n] = (uint32_t)x[n] & mask;
     lds_read_bucket<B>(tab, j[n], v[n]);
   }
-  uint32_t hits = 0;
   if (!displaced) {  // (wave-uniform) every key sits in its home bucket
 #pragma unroll
-    for (int n = 0; n < N; ++n) {
-      uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
-#pragma unroll
-      for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
-      hits += (uint32_t)(act[n] && m == 0u);
-    }
+    for (int n = 0; n < N; ++n) hits += (uint32_t)(bucket_has<B>(v[n], x[n]) & act[n]);
     return hits;
   }
   bool pend[N];
   bool any = false;
 #pragma unroll
   for (int n = 0; n < N; ++n) {
-    uint32_t m = (uint32_t)(v[n][0] ^ x[n]);
-#pragma unroll
-    for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(v[n][k] ^ x[n]));
-    const bool hit = m == 0u;
+    const bool hit = bucket_has<B>(v[n], x[n]);
     hits += (uint32_t)(hit && act[n]);
     pend[n] = act[n] && !hit && v[n][B - 1] != kEmpty;
     any |= pend[n];
@@ -339,11 +351,9 @@ __device__ __forceinline__ uint32_t lds_lookup(const int32_t* tab, uint32_t L, b
         jj = (jj + 1u) & mask;
         int32_t w[B];
         lds_read_bucket<B>(tab, jj, w);
-        uint32_t m = (uint32_t)(w[0] ^ x[n]);
-#pragma unroll
-        for (int k = 1; k < B; ++k) m = min(m, (uint32_t)(w[k] ^ x[n]));
-        hits += (uint32_t)(p && m == 0u);
-        p = p && m != 0u && w[B - 1] != kEmpty;
+        const bool hit = bucket_has<B>(w, x[n]);
+        hits += (uint32_t)(p && hit);
+        p = p && !hit && w[B - 1] != kEmpty;
       }
       bad |= p;
     }
@@ -551,6 +561,228 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
   return hits;
 }
 
+// ---------------------------------------------------------------- ordered chunk tables
+// Wave-wide max steps through DPP (gfx9 row shifts and row broadcasts): lanes
+// with no source lane keep `ident`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int32_t dpp_from(int32_t x, int32_t ident) {
+  return __builtin_amdgcn_update_dpp(ident, x, CTRL, ROWS, 0xf, false);
+}
+
+// Inclusive prefix max over the lanes of the wave (lane order).
+__device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
+  constexpr int32_t I = INT32_MIN;
+  x = max(x, dpp_from<0x111, 0xf>(x, I));  // row_shr:1
+  x = max(x, dpp_from<0x112, 0xf>(x, I));  // row_shr:2
+  x = max(x, dpp_from<0x114, 0xf>(x, I));  // row_shr:4
+  x = max(x, dpp_from<0x118, 0xf>(x, I));  // row_shr:8
+  x = max(x, dpp_from<0x142, 0xa>(x, I));  // row_bcast:15 into rows 1 and 3
+  x = max(x, dpp_from<0x143, 0xc>(x, I));  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
+// The value of lane - 1 (lane 0 gets `ident`): DPP wave_shr:1.
+__device__ __forceinline__ int32_t wave_shr1(int32_t x, int32_t ident) {
+  return dpp_from<0x138, 0xf>(x, ident);
+}
+
+// Order-preserving bucket of a chunk table: the chunk's value range
+// [first, last] spread over nb buckets, h(x) = clamp(x * scale + bias) with
+// one fused multiply-add.  Every step is monotone (conversion, a multiply-add
+// with positive scale rounded to nearest, a clamp, truncation), so ascending
+// keys have non-decreasing buckets; keys outside the range land in bucket 0
+// or nb - 1, where they cannot match (they are not in the chunk).
+struct OrderedHash {
+  float scale, bias, top_f;  // top_f = nb - 1
+};
+
+__device__ __forceinline__ uint32_t ordered_bucket(uint32_t x, const OrderedHash& oh) {
+  const float f = __builtin_fmaf((float)x, oh.scale, oh.bias);
+  return (uint32_t)__builtin_amdgcn_fmed3f(f, 0.f, oh.top_f);
+}
+
+// Lookups of N keys per lane in an ordered chunk table: the home bucket, then
+// the next `D` buckets (D = the chunk's largest displacement, wave-uniform,
+// 0 for most chunks), the last clamped to the region's last bucket `rmax`.
+// Slots hold the current chunk's keys or keys of earlier chunks of the same
+// pair, which all lie below every key probed here, so a slot equal to x is x.
+template <int B, int N>
+__device__ __forceinline__ uint32_t ordered_lookup(const int32_t* tab, const OrderedHash& oh, int32_t D,
+                                                   uint32_t rmax, const int32_t (&x)[N],
+                                                   const bool (&in)[N]) {
+  int32_t v[N][B];
+  uint32_t j[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    j[n] = ordered_bucket((uint32_t)x[n], oh);
+    lds_read_bucket<B>(tab, j[n], v[n]);
+  }
+  uint32_t hits = 0;
+#pragma unroll
+  for (int n = 0; n < N; ++n) hits += (uint32_t)(bucket_has<B>(v[n], x[n]) & in[n]);
+  for (int32_t d = 1; d <= D; ++d) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      const uint32_t jj = min(j[n] + (uint32_t)d, rmax);
+      int32_t w[B];
+      lds_read_bucket<B>(tab, jj, w);
+      hits += (uint32_t)(bucket_has<B>(w, x[n]) & in[n]);
+    }
+  }
+  return hits;
+}
+
+// A pair whose smaller set S exceeds the bin's chunk, both sets ascending:
+// the same stream as pair_chunked_wave (S staged in 256-element groups
+// through the ring, G in aligned blocks, a block reaching past the current
+// chunk probed again after the switch), with each group of S as one chunk
+// and its table built without atomics.  The chunk's keys get ordered buckets
+// (ordered_bucket over the chunk's range, nb buckets of B slots), and key i
+// (in set order) takes slot pos_i = max(B*h_i, pos_{i-1} + 1): where a
+// sequential linear-probing insert in set order would put it.  That is
+// pos_i = i + max_{j<=i}(B*h_j - j), a prefix max over the wave.  No chunk is
+// cleared before the next is built: the slots it leaves hold keys below
+// everything the next chunk is asked about.  The region (nb*B + 256 slots)
+// is filled with EMPTY once, when the pair ends, for the next pair.
+template <int B, int U>
+__device__ __forceinline__ uint32_t pair_ordered_wave(const int32_t* __restrict__ S, int32_t ns,
+                                                      const int32_t* __restrict__ G, int32_t ng,
+                                                      int32_t* tab, int32_t* ring, uint32_t nb,
+                                                      uint32_t& unordered) {
+  constexpr int32_t kBlock = 4 * U * kWave;
+  const int lane = __lane_id();
+  const uint32_t rmax = nb + (uint32_t)(kGroupElems / B) - 1u;  // last bucket of the region
+  const int32_t s0 = block_origin(S);
+  const int32_t s_last4 = s0 + ((ns - 1 - s0) & ~3);
+  auto group_load = [&](int32_t g) {
+    const int32_t e = s0 + g * kGroupElems + 4 * lane;
+    return *reinterpret_cast<const v4i*>(S + (e < s_last4 ? e : s_last4));
+  };
+  auto group_store = [&](int32_t g, const v4i& v) {
+    *reinterpret_cast<v4i*>(ring + (g & 1) * kGroupElems + 4 * lane) = v;
+  };
+  auto ring_at = [&](int32_t idx) { return ring[(idx - s0) & (kRing - 1)]; };
+  {
+    const v4i g0 = group_load(0), g1 = group_load(1);
+    group_store(0, g0);
+    group_store(1, g1);
+  }
+  int32_t sc = 2;  // next group to stage
+  // the current chunk: group gk, element indices [cb, ce)
+  int32_t gk = 0;
+  int32_t cb = 0, ce = s0 + kGroupElems < ns ? s0 + kGroupElems : ns;
+  bool last = ce >= ns;
+  // the chunk answers for the values [lo, top] (see pair_chunked_wave)
+  uint32_t lo = 0u, top = 0u;
+  OrderedHash oh{0.f, 0.f, (float)(nb - 1u)};
+  int32_t D = 0;
+  auto build = [&]() {
+    while (sc <= gk) {
+      __builtin_amdgcn_wave_barrier();
+      group_store(sc, group_load(sc));
+      ++sc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const v4i kv = *reinterpret_cast<const v4i*>(ring + (gk & 1) * kGroupElems + 4 * lane);
+    const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(cb));
+    const uint32_t last_el = (uint32_t)__builtin_amdgcn_readfirstlane(ring_at(ce - 1));
+    // nb buckets over [first, last_el]; a range that wraps (keys out of
+    // order, flagged below) gets some valid table
+    const float span = (float)(last_el - first) + 1.0f;
+    oh.scale = (float)nb * __builtin_amdgcn_rcpf(span);  // (any scale works: build and lookups share it)
+    oh.bias = -(float)first * oh.scale;
+    const int32_t e0 = s0 + gk * kGroupElems + 4 * lane;  // element index of kv.x
+    const int32_t k[4] = {kv.x, kv.y, kv.z, kv.w};
+    bool act[4];
+    int32_t h[4], p[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      act[c] = e0 + c >= cb && e0 + c < ce;
+      h[c] = (int32_t)ordered_bucket((uint32_t)k[c], oh);
+      const int32_t t = act[c] ? B * h[c] - (4 * lane + c) : INT32_MIN / 2;
+      p[c] = c == 0 ? t : max(p[c - 1], t);
+    }
+    const int32_t before = wave_shr1(wave_prefix_max(p[3]), INT32_MIN / 2);
+    int32_t disp = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t pos = 4 * lane + c + max(before, p[c]);
+      if (act[c]) {
+        tab[pos] = k[c];
+        disp = max(disp, pos / B - h[c]);
+      }
+    }
+    D = __any(disp > 0) ? __builtin_amdgcn_readlane(wave_prefix_max(disp), kWave - 1) : 0;
+    // strictly ascending keys, above the previous chunk's range
+    const uint32_t prev = (uint32_t)wave_shr1(kv.w, 0);
+    uint32_t bad_order = (uint32_t)(act[0] && e0 - 1 >= cb && (uint32_t)k[0] <= prev);
+#pragma unroll
+    for (int c = 1; c < 4; ++c) bad_order |= (uint32_t)(act[c] && act[c - 1] && (uint32_t)k[c] <= (uint32_t)k[c - 1]);
+    unordered |= bad_order | (uint32_t)(first < lo);
+    __builtin_amdgcn_wave_barrier();
+    top = last ? 0xFFFFFFFFu : last_el;
+  };
+  build();
+  int32_t pos = block_origin(G);
+  GBlock<U> b0, b1, b2;
+  load_block<U>(G, pos, 0, ng, b0);
+  load_block<U>(G, pos + kBlock, 0, ng, b1);
+  uint32_t hits = 0;
+  auto step = [&](GBlock<U>& cur, GBlock<U>& ahead) -> bool {
+    const int32_t staged_g = sc;
+    const v4i staged = group_load(staged_g);          // unconditional, as in pair_chunked_wave
+    load_block<U>(G, pos + 2 * kBlock, 0, ng, ahead);  // unconditional
+#pragma unroll
+    for (int i = 0; i < 4 * U; ++i) unordered |= (uint32_t)cur.act[i] & (uint32_t)((uint32_t)cur.x[i] < lo);
+    while (true) {
+      bool in[4 * U];
+      bool rest = false, any_in = false;
+#pragma unroll
+      for (int i = 0; i < 4 * U; ++i) {
+        const bool below = (uint32_t)cur.x[i] <= top;
+        in[i] = cur.act[i] && below;
+        cur.act[i] = cur.act[i] && !below;
+        rest |= cur.act[i];
+        any_in |= in[i];
+      }
+      if (__any(any_in)) hits += ordered_lookup<B, 4 * U>(tab, oh, D, rmax, cur.x, in);
+      if (last || !__any(rest)) break;
+      // the block reaches past this chunk: build the next one
+      lo = top + 1u;
+      ++gk;
+      cb = s0 + gk * kGroupElems;
+      ce = ns - cb < kGroupElems ? ns : cb + kGroupElems;
+      last = ce >= ns;
+      build();
+    }
+    if (pos + kBlock >= ng) return false;
+    if (staged_g == sc && sc <= gk + 1) {
+      __builtin_amdgcn_wave_barrier();
+      group_store(sc, staged);
+      ++sc;
+    }
+    pos += kBlock;
+    return true;
+  };
+  while (step(b0, b2) && step(b1, b0) && step(b2, b1)) {
+  }
+  // leave the region all EMPTY for the next pair
+  __builtin_amdgcn_wave_barrier();
+  const int32_t region4 = (int32_t)((nb * (uint32_t)B + kGroupElems + 3u) >> 2);
+  for (int32_t s4 = lane; s4 < region4; s4 += kWave)
+    *reinterpret_cast<v4i*>(tab + 4 * s4) = v4i{kEmpty, kEmpty, kEmpty, kEmpty};
+  __builtin_amdgcn_wave_barrier();
+  return hits;
+}
+
+// Buckets of an ordered chunk table for a bin: calculate_length(256, alpha,
+// B), capped so that nb*B + 256 slots fit the wave's table region.
+__device__ __forceinline__ uint32_t ordered_nb(float alpha, int32_t B, int32_t slots_per_wave) {
+  const int32_t want = calc_length(kGroupElems, alpha, B);
+  const int32_t cap = (slots_per_wave - kGroupElems) / B;
+  return (uint32_t)(want > 0 && want < cap ? want : cap);
+}
+
 // Persistent kernel over the size-sorted order (largest first).  Static
 // schedule: wave w takes the positions w, w + W, w + 2W, ..., so every wave
 // gets a near-equal mix and no global atomic is needed.  Work stealing
@@ -563,8 +795,11 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
 // of 4 (or of the requested length, if shorter) is dealt statically.
 // Either way the next pair's offsets are fetched before the current
 // pair is processed.
+// The default launch keeps 16 waves on a CU (4 per SIMD, the LDS split), so
+// the kernel is held to 4 waves per SIMD worth of registers (128 VGPRs);
+// with U = 4 it needs more and is held to 2.
 template <int B, int U, int CPL, class Src>
-__global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
+__global__ void __launch_bounds__(kPairBlock, (U > 2 ? 2 : 4)) pairs_batched_kernel(
     const Src src, const PairRange rg, const int32_t* __restrict__ order_all, int32_t* ctl,
     int32_t* __restrict__ sizes, const PairsConst cfg, int32_t slots_per_wave) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_dyn[];
@@ -669,11 +904,13 @@ __global__ void __launch_bounds__(kPairBlock) pairs_batched_kernel(
       if (ns <= bin.chunk)
         hits = wide ? pair_resident_wave<B, U, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad)
                     : pair_resident_wave<B, 1, CPL>(S, ns, Gp, ng, tab, bin.alpha, bad);
-      else
-        hits = wide ? pair_chunked_wave<B, U>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad,
-                                              unordered)
-                    : pair_chunked_wave<B, 1>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad,
-                                              unordered);
+      else if ((int64_t)ng >= (int64_t)kOrderedRatio * ns)  // (wide: kOrderedRatio >= kWideRatio)
+        hits = pair_chunked_wave<B, U>(S, ns, Gp, ng, tab, ring, bin.alpha, bin.chunk, bad, unordered);
+      else {
+        const uint32_t nb = ordered_nb(bin.alpha, B, slots_per_wave);
+        // (one block per lane: a wider block crosses more chunk boundaries)
+        hits = pair_ordered_wave<B, 1>(S, ns, Gp, ng, tab, ring, nb, unordered);
+      }
     }
     const uint32_t total = wave_sum(hits);
     if (lane == 0) sizes[i] = (int32_t)total;

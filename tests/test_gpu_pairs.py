@@ -332,6 +332,101 @@ def test_batched_flags_unsorted_sets_on_the_chunked_path(api, oracle_lib):
     plan.close()
 
 
+def _clustered_set(rng, n, lo, hi):
+    """n distinct ascending values in [lo, hi]: runs of consecutive integers
+    at a few random places, so a 256-key chunk's values are bunched into a
+    sliver of its range."""
+    vals = set()
+    while len(vals) < n:
+        start = int(rng.integers(lo, hi))
+        run = int(rng.integers(1, 200))
+        vals.update(range(start, min(hi + 1, start + run)))
+    return np.array(sorted(vals)[:n], np.int64)
+
+
+def test_ordered_chunk_tables_clustered_and_extreme_values(api, oracle_lib):
+    """Pairs whose sets are within 8x of each other and larger than the
+    chunk take ordered chunk tables.  Values bunched into runs of consecutive
+    integers put many keys into few buckets (long displacements, the walk
+    past the home bucket); values 0 and 2^31 - 1 sit at the ends of the
+    order-preserving hash; sets start at every offset mod 4 within their
+    arrays (the staging groups' leading inactive lanes).  Each size equals
+    the oracle's, with the roles of the two sets swapped as well."""
+    rng = np.random.default_rng(77)
+    a_parts, b_parts = [], []
+    for i in range(48):
+        ns = int(rng.integers(300, 3000))
+        ng = int(ns * rng.uniform(1.0, 7.5))
+        if i % 3 == 0:
+            S = _clustered_set(rng, ns, 0, 1 << 20)
+            G_ = np.union1d(S[rng.random(ns) < 0.4], _clustered_set(rng, ng, 0, 1 << 20))[:ng]
+        elif i % 3 == 1:
+            S = np.unique(np.concatenate([[0, (1 << 31) - 1], rng.choice(1 << 31, ns, replace=False)]))
+            G_ = np.unique(np.concatenate([[0, (1 << 31) - 1], S[rng.random(len(S)) < 0.3],
+                                           rng.choice(1 << 31, ng, replace=False)]))
+        else:  # consecutive integers: the chunk's range is exactly its size
+            base = int(rng.integers(0, 1 << 24))
+            S = np.arange(base, base + ns, dtype=np.int64)
+            G_ = np.unique(np.concatenate([S[::3], np.arange(base + ns, base + ns + ng // 2)]))
+        a_parts.append(S)
+        b_parts.append(np.sort(G_))
+    # pad so that the sets start at every offset mod 4 within their arrays
+    pad_a = [np.array([7] * (i % 4), np.int64) for i in range(len(a_parts))]
+    A = np.concatenate([x for pa, s_ in zip(pad_a, a_parts) for x in (pa, s_)])
+    starts = np.cumsum([0] + [len(pa) + len(s_) for pa, s_ in zip(pad_a, a_parts)])[:-1] + \
+        np.array([len(pa) for pa in pad_a])
+    a_off_lo = starts.astype(np.int64)
+    a_off_hi = (starts + np.array([len(s_) for s_ in a_parts])).astype(np.int64)
+    # per-pair CSR over a copy with the pads: pair i = A[a_off_lo[i]:a_off_hi[i]]
+    Ac = np.concatenate([A[lo_:hi_] for lo_, hi_ in zip(a_off_lo, a_off_hi)])
+    assert np.array_equal(Ac, np.concatenate(a_parts))
+    b_off = np.concatenate([[0], np.cumsum([len(x) for x in b_parts])]).astype(np.int64)
+    Bc = np.concatenate(b_parts)
+    a_off = np.concatenate([[0], np.cumsum([len(x) for x in a_parts])]).astype(np.int64)
+    want = np.array([len(np.intersect1d(x, y)) for x, y in zip(a_parts, b_parts)], np.int32)
+    assert np.array_equal(oracle_lib.pairs_hash(a_off, Ac, b_off, Bc), want)
+    plan = api.PairsPlan(len(want))
+    # unaligned starts: the A family read in place from the padded array, one
+    # pair per call (its offsets index the padded array)
+    dA = torch.from_numpy(A.astype(np.int32)).cuda()
+    for i in range(len(want)):
+        ao = torch.tensor([a_off_lo[i], a_off_hi[i]], dtype=torch.int64, device="cuda")
+        bo = torch.tensor([b_off[i], b_off[i + 1]], dtype=torch.int64, device="cuda")
+        dB = torch.from_numpy(Bc.astype(np.int32)).cuda()
+        got = plan.run(ao, dA, bo, dB).cpu().tolist()
+        assert got == [int(want[i])], (i, got, int(want[i]))
+    # the whole batch, both role orders
+    got = plan.run(*_dev(a_off, Ac, b_off, Bc)).cpu().numpy()
+    assert np.array_equal(got, want)
+    got = plan.run(*_dev(b_off, Bc, a_off, Ac)).cpu().numpy()
+    assert np.array_equal(got, want)
+    plan.close()
+
+
+def test_ordered_chunk_flags_unsorted_or_duplicated_keys(api):
+    """Two swapped neighbours or a repeated key inside one chunk of an
+    ordered-path pair are reported (RS_EINVAL), not counted."""
+    from reset_amd.lib import RS_EINVAL, ResetError
+
+    rng = np.random.default_rng(91)
+    S = np.sort(rng.choice(1 << 24, 2000, replace=False))
+    G_ = np.sort(rng.choice(1 << 24, 5000, replace=False))
+    off_s = np.array([0, len(S)], np.int64)
+    off_g = np.array([0, len(G_)], np.int64)
+    plan = api.PairsPlan(1)
+    assert plan.run(*_dev(off_s, S, off_g, G_)).cpu().tolist() == [len(np.intersect1d(S, G_))]
+    for bad in ("swap", "dup"):
+        T = S.copy()
+        if bad == "swap":
+            T[1000], T[1001] = T[1001], T[1000]
+        else:
+            T[1001] = T[1000]
+        with pytest.raises(ResetError) as e:
+            plan.run(*_dev(off_s, T, off_g, G_))
+        assert e.value.status == RS_EINVAL
+    plan.close()
+
+
 def _mixed_opts(api):
     """Bins with different bucket sizes and lanes per pair: bins 0-3 on 16
     lanes (b 1), 4-5 on 32 lanes (b 2), 6 on 16 lanes (b 4), 7-9 a wave with
