@@ -20,57 +20,216 @@ rs_status require_layout(const rs_tables* t, int32_t layout, const int32_t** tab
   return RS_OK;
 }
 
-// hashtable.cu:53-101, both layouts in one pass over (row, element).  One
-// thread per CSR entry; CAS(-1 -> v) walks slots k = 0..b-1 of the home
-// bucket, then the next bucket mod L, bounded at L*b attempts.  Conflicts
-// (failed CAS) are summed per wave and added once.
-template <bool NORMAL, bool HIER>
-__global__ void __launch_bounds__(256) build_kernel(const int32_t* __restrict__ rows,
-                                                    const int32_t* __restrict__ elems, int64_t E,
-                                                    const int32_t* __restrict__ Ls,
-                                                    const int64_t* __restrict__ toff, int64_t BN,
-                                                    int32_t b, int32_t M, int32_t* tn, int32_t* th,
-                                                    int32_t* status,
-                                                    unsigned long long* conflicts) {
-  uint32_t cn = 0, ch = 0;
-  bool full = false, bad = false;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t u = rows[e];
-    const int32_t v = elems[e];
-    if (v == kEmpty) {
-      bad = true;
-      continue;
+// ---------------------------------------------------------------- build (a4 + a6)
+// hashtable.cu:53-101's insert: CAS(-1 -> v) walks slots k = 0..b-1 of the
+// home bucket, then the next bucket mod L, bounded at L*b attempts; failed
+// CAS are counted as conflicts.  Here the walk runs in LDS.  Consecutive
+// tables occupy consecutive buckets of every slot plane, so a batch of
+// consecutive small tables is a single run of W buckets in each of the b
+// planes: a workgroup fills an LDS image of those b runs with EMPTY, inserts
+// the batch's elements into it with LDS atomics, and writes the b runs out
+// with 16-byte stores.  HBM sees the keys once and every table slot once,
+// where an insert into HBM writes a cache line back per atomic (and needs a
+// separate EMPTY fill before it).  Tables too large for a batch are filled
+// and inserted in HBM, one workgroup per table.
+constexpr int kBuildBlock = 256;           // large tables
+constexpr int kBuildLdsBlock = 512;        // batches: 8 waves
+constexpr int32_t kBuildLdsSlots = 10240;  // 40 KiB per workgroup: 4 workgroups on a CU
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+
+BuildGeom build_geom(int32_t b) {
+  BuildGeom g;
+  g.cap = (kBuildLdsSlots / b) & ~7;  // buckets of one LDS plane
+  g.lsmall = 8;                       // largest power of two <= cap / 2
+  while (g.lsmall * 2 <= g.cap / 2) g.lsmall *= 2;
+  g.wwin = g.cap - g.lsmall;
+  return g;
+}
+
+// The walk of one key from its home bucket in table `tab` (slot k of bucket j
+// at tab[j + k * stride]).  Returns false if L*b attempts found no EMPTY slot.
+__device__ __forceinline__ bool insert_walk(int32_t* tab, int64_t stride, uint32_t L, int32_t b,
+                                            uint32_t bucket, int32_t v, uint32_t& conflicts) {
+  int32_t k = 0;
+  const uint32_t bound = L * (uint32_t)b;
+  uint32_t tries = 0;
+  while (atomicCAS(&tab[bucket + (int64_t)k * stride], kEmpty, v) != kEmpty) {
+    if (++tries >= bound) return false;
+    ++conflicts;
+    if (++k == b) {
+      k = 0;
+      bucket = (bucket + 1u) & (L - 1u);
     }
-    const uint32_t L = (uint32_t)Ls[u];
-    const int64_t base = toff[u];
-    const uint32_t bound = L * (uint32_t)b;
+  }
+  return true;
+}
+
+// Batch boundaries: a batch is a run of consecutive small tables (L <=
+// lsmall) whose first buckets fall in one window of wwin buckets, so that it
+// spans at most wwin + lsmall = cap buckets; a table above lsmall is a batch
+// of its own, built in LDS up to cap buckets and in HBM beyond (`large`).
+__global__ void batch_flags_kernel(const int32_t* __restrict__ Ls, const int64_t* __restrict__ toff,
+                                   int64_t n, int32_t lsmall, int32_t cap, int64_t wwin,
+                                   int32_t* __restrict__ first, int32_t* __restrict__ large) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const bool alone = Ls[t] > lsmall;
+    bool start = t == 0 || alone;
+    if (!start) start = Ls[t - 1] > lsmall || toff[t] / wwin != toff[t - 1] / wwin;
+    first[t] = start;
+    large[t] = Ls[t] > cap;
+  }
+}
+
+__global__ void batch_scatter_kernel(const int32_t* __restrict__ first, const int64_t* __restrict__ fpos,
+                                     const int32_t* __restrict__ large, const int64_t* __restrict__ lpos,
+                                     int64_t n, int32_t* __restrict__ batch_first,
+                                     int32_t* __restrict__ large_list) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (first[t]) batch_first[fpos[t]] = (int32_t)t;
+    if (large[t]) large_list[lpos[t]] = (int32_t)t;
+    if (t == 0) batch_first[fpos[n]] = (int32_t)n;
+  }
+}
+
+// One workgroup per batch at a time.  The image is all EMPTY when a batch
+// starts: it is filled once, and the write-out puts EMPTY back behind every
+// slot it copies.  The next batch's keys (up to kBuildPre per thread) are
+// loaded before the write-out, so their latency hides behind the stores.
+constexpr int kBuildPre = 2;
+
+// A workgroup barrier that waits for this thread's LDS operations only: the
+// write-out's global stores go to slots no later batch reads, so they need
+// not drain before the next batch starts.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int LAYOUT>
+__global__ void __launch_bounds__(kBuildLdsBlock) build_lds_kernel(
+    const int32_t* __restrict__ batch_first, int32_t nbatches, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ rows, const int32_t* __restrict__ elems, const int32_t* __restrict__ Ls,
+    const int64_t* __restrict__ toff, int64_t BN, int32_t b, int32_t M, int32_t cap,
+    int32_t* __restrict__ T, int32_t* status, unsigned long long* conflicts) {
+  extern __shared__ __attribute__((aligned(16))) int32_t img[];  // b planes of cap slots
+  uint32_t conf = 0;
+  bool full = false, bad = false;
+  const v4i_t empty4 = {kEmpty, kEmpty, kEmpty, kEmpty};
+  const int32_t tid = (int32_t)threadIdx.x;
+  for (int32_t i = 4 * tid; i < cap * b; i += 4 * kBuildLdsBlock)
+    *reinterpret_cast<v4i_t*>(img + i) = empty4;
+  // the batch `bi`: tables [t0, t1), elements [e0, e1), buckets from `base`;
+  // `small` is false for a large table (skipped, block-uniform)
+  int32_t t0 = 0, t1 = 0, e0 = 0, e1 = 0;
+  int64_t base = 0;
+  bool small = false;
+  int32_t pu[kBuildPre], pv[kBuildPre];
+  auto fetch = [&](int32_t bi) {
+    t0 = batch_first[bi];
+    t1 = batch_first[bi + 1];
+    small = Ls[t0] <= cap;
+    base = toff[t0];
+    e0 = off[t0];
+    e1 = off[t1];
 #pragma unroll
-    for (int which = 0; which < 2; ++which) {
-      if (which == 0 && !NORMAL) continue;
-      if (which == 1 && !HIER) continue;
-      int32_t* T = (which == 0 ? tn : th) + base;
-      uint32_t bucket = which == 0 ? hash_normal((uint32_t)v, L) : hash_hier((uint32_t)v, L, (uint32_t)M);
-      int32_t k = 0;
-      uint32_t tries = 0;
-      while (atomicCAS(&T[bucket + (int64_t)k * BN], kEmpty, v) != kEmpty) {
-        if (++tries >= bound) {
-          full = true;
-          break;
-        }
-        if (which == 0) ++cn; else ++ch;
-        if (++k == b) {
-          k = 0;
-          bucket = (bucket + 1u) & (L - 1u);
-        }
+    for (int r = 0; r < kBuildPre; ++r) {
+      const int32_t e = e0 + tid + r * kBuildLdsBlock;
+      pu[r] = 0;
+      pv[r] = 0;
+      if (small && e < e1) {
+        pu[r] = rows[e];
+        pv[r] = elems[e];
       }
     }
+  };
+  auto insert = [&](int32_t u, int32_t v) {
+    if (v == kEmpty) {
+      bad = true;
+      return;
+    }
+    const uint32_t L = (uint32_t)Ls[u];
+    int32_t* tab = img + (int32_t)(toff[u] - base);
+    full |= !insert_walk(tab, cap, L, b, home_bucket<LAYOUT>((uint32_t)v, L, (uint32_t)M), v, conf);
+  };
+  int32_t bi = blockIdx.x;
+  if (bi < nbatches) fetch(bi);
+  lds_barrier();
+  while (bi < nbatches) {
+    const int32_t nbi = bi + (int32_t)gridDim.x;
+    if (!small) {  // a large table: built in HBM
+      if (nbi < nbatches) fetch(nbi);
+      bi = nbi;
+      continue;
+    }
+#pragma unroll
+    for (int r = 0; r < kBuildPre; ++r)
+      if (e0 + tid + r * kBuildLdsBlock < e1) insert(pu[r], pv[r]);
+    for (int32_t e = e0 + tid + kBuildPre * kBuildLdsBlock; e < e1; e += kBuildLdsBlock)
+      insert(rows[e], elems[e]);
+    lds_barrier();
+    const int64_t obase = base;
+    const int32_t W = (int32_t)(toff[t1] - obase);  // a multiple of 8, <= cap
+    if (nbi < nbatches) fetch(nbi);
+    for (int32_t k = 0; k < b; ++k) {
+      int32_t* dst = T + obase + (int64_t)k * BN;
+      for (int32_t i = 4 * tid; i < W; i += 4 * kBuildLdsBlock) {
+        v4i_t* s4 = reinterpret_cast<v4i_t*>(img + k * cap + i);
+        *reinterpret_cast<v4i_t*>(dst + i) = *s4;
+        *s4 = empty4;
+      }
+    }
+    lds_barrier();
+    bi = nbi;
   }
-  const uint32_t sn = wave_sum(cn), sh = wave_sum(ch);
-  if (__lane_id() == 0) {
-    if (sn) atomicAdd(&conflicts[0], (unsigned long long)sn);
-    if (sh) atomicAdd(&conflicts[1], (unsigned long long)sh);
+  const uint32_t sc = wave_sum(conf);
+  if (__lane_id() == 0 && sc) atomicAdd(conflicts, (unsigned long long)sc);
+  if (__any(full) && __lane_id() == 0) atomicOr(status, kErrTableFull);
+  if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
+}
+
+// The large tables: EMPTY into their b runs, then (next kernel) their
+// elements inserted in HBM; one workgroup per table.
+__global__ void __launch_bounds__(kBuildBlock) fill_large_kernel(const int32_t* __restrict__ list, int32_t nlist,
+                                                                 const int32_t* __restrict__ Ls,
+                                                                 const int64_t* __restrict__ toff, int64_t BN,
+                                                                 int32_t b, int32_t* __restrict__ T) {
+  const v4i_t empty4 = {kEmpty, kEmpty, kEmpty, kEmpty};
+  for (int32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+    const int32_t t = list[i];
+    const int32_t L = Ls[t];
+    for (int32_t k = 0; k < b; ++k) {
+      int32_t* dst = T + toff[t] + (int64_t)k * BN;
+      for (int32_t j = 4 * (int32_t)threadIdx.x; j < L; j += 4 * kBuildBlock)
+        *reinterpret_cast<v4i_t*>(dst + j) = empty4;
+    }
   }
+}
+
+template <int LAYOUT>
+__global__ void __launch_bounds__(kBuildBlock) insert_large_kernel(
+    const int32_t* __restrict__ list, int32_t nlist, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ elems, const int32_t* __restrict__ Ls, const int64_t* __restrict__ toff,
+    int64_t BN, int32_t b, int32_t M, int32_t* __restrict__ T, int32_t* status,
+    unsigned long long* conflicts) {
+  uint32_t conf = 0;
+  bool full = false, bad = false;
+  for (int32_t i = blockIdx.x; i < nlist; i += gridDim.x) {
+    const int32_t t = list[i];
+    const uint32_t L = (uint32_t)Ls[t];
+    int32_t* tab = T + toff[t];
+    for (int32_t e = off[t] + (int32_t)threadIdx.x; e < off[t + 1]; e += kBuildBlock) {
+      const int32_t v = elems[e];
+      if (v == kEmpty) {
+        bad = true;
+        continue;
+      }
+      full |= !insert_walk(tab, BN, L, b, home_bucket<LAYOUT>((uint32_t)v, L, (uint32_t)M), v, conf);
+    }
+  }
+  const uint32_t sc = wave_sum(conf);
+  if (__lane_id() == 0 && sc) atomicAdd(conflicts, (unsigned long long)sc);
   if (__any(full) && __lane_id() == 0) atomicOr(status, kErrTableFull);
   if (__any(bad) && __lane_id() == 0) atomicOr(status, kErrBadInput);
 }
@@ -108,40 +267,71 @@ __global__ void __launch_bounds__(256) probe_order_kernel(const int32_t* __restr
   }
 }
 
-// Fills n4 16-byte groups with EMPTY (a kernel rather than a memset node, so
-// the sequence stays capturable).
-__global__ void fill_empty_kernel(int4* __restrict__ p, int64_t n4) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = int4{kEmpty, kEmpty, kEmpty, kEmpty};
+// The batches of a handle (once per build): the first table of every batch
+// and the list of large tables.
+static rs_status plan_batches(rs_tables* t, hipStream_t st) {
+  t->geom = build_geom(t->b);
+  t->nbatches = 0;
+  t->nlarge = 0;
+  if (t->n == 0) return RS_OK;
+  const int64_t n = t->n;
+  DevBuf first, large, fpos, lpos, tmp;
+  RS_TRY(first.alloc(sizeof(int32_t) * (size_t)n));
+  RS_TRY(large.alloc(sizeof(int32_t) * (size_t)n));
+  RS_TRY(fpos.alloc(sizeof(int64_t) * (size_t)(n + 1)));
+  RS_TRY(lpos.alloc(sizeof(int64_t) * (size_t)(n + 1)));
+  const int g = grid_for(n, 256, 16);
+  batch_flags_kernel<<<g, 256, 0, st>>>(t->lengths.as<int32_t>(), t->toff.as<int64_t>(), n, t->geom.lsmall,
+                                        t->geom.cap, t->geom.wwin, first.as<int32_t>(), large.as<int32_t>());
+  RS_HIP(hipGetLastError());
+  RS_TRY(exclusive_scan_i32_to_i64(first.as<int32_t>(), n, fpos.as<int64_t>(), tmp, st));
+  RS_TRY(exclusive_scan_i32_to_i64(large.as<int32_t>(), n, lpos.as<int64_t>(), tmp, st));
+  int64_t counts[2] = {0, 0};
+  RS_HIP(hipMemcpyAsync(&counts[0], fpos.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipMemcpyAsync(&counts[1], lpos.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RS_HIP(hipStreamSynchronize(st));
+  t->nbatches = (int32_t)counts[0];
+  t->nlarge = (int32_t)counts[1];
+  RS_TRY(t->batch_first.alloc(sizeof(int32_t) * (size_t)(t->nbatches + 1)));
+  RS_TRY(t->large_list.alloc(sizeof(int32_t) * (size_t)(t->nlarge > 0 ? t->nlarge : 1)));
+  batch_scatter_kernel<<<g, 256, 0, st>>>(first.as<int32_t>(), fpos.as<int64_t>(), large.as<int32_t>(),
+                                          lpos.as<int64_t>(), n, t->batch_first.as<int32_t>(),
+                                          t->large_list.as<int32_t>());
+  RS_HIP(hipGetLastError());
+  RS_HIP(hipStreamSynchronize(st));  // the temporaries are released on return
+  return RS_OK;
 }
 
-// a4 + a6 for the layouts in `mask` (tables allocated): EMPTY fill, then the
-// insert of every (row, element).  Conflicts are added to the counters at
-// status+16 (normal) and status+24 (hierarchical).
-static rs_status fill_and_insert(rs_tables* t, int32_t mask, hipStream_t st) {
-  // BN is a multiple of 8 (every length is a power of two >= 8), so a plane
-  // holds whole 16-byte groups
-  const int64_t n4 = t->BN * (int64_t)t->b / 4;
-  const int fg = grid_for(n4, 256, 16);
-  if (mask & RS_BUILD_NORMAL) fill_empty_kernel<<<fg, 256, 0, st>>>(t->t_normal.as<int4>(), n4);
-  if (mask & RS_BUILD_HIERARCHICAL) fill_empty_kernel<<<fg, 256, 0, st>>>(t->t_hier.as<int4>(), n4);
-  if (t->E > 0) {
-    auto* conf = reinterpret_cast<unsigned long long*>(t->status.as<char>() + 16);
-    const int grid = grid_for(t->E, 256, 16);
-    if (mask == RS_BUILD_BOTH)
-      build_kernel<true, true><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, t->b, t->M, t->t_normal.as<int32_t>(), t->t_hier.as<int32_t>(), t->d_status(), conf);
-    else if (mask == RS_BUILD_NORMAL)
-      build_kernel<true, false><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, t->b, t->M, t->t_normal.as<int32_t>(), nullptr, t->d_status(), conf);
-    else
-      build_kernel<false, true><<<grid, 256, 0, st>>>(
-          t->rows.as<int32_t>(), t->d_elems, t->E, t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
-          t->BN, t->b, t->M, nullptr, t->t_hier.as<int32_t>(), t->d_status(), conf);
+template <int LAYOUT>
+static void build_layout(rs_tables* t, int32_t* T, unsigned long long* conf, hipStream_t st) {
+  const BuildGeom& g = t->geom;
+  if (t->nbatches > 0) {
+    int blocks = device_cu_count() * 4;
+    if (blocks > t->nbatches) blocks = t->nbatches;
+    const size_t lds = sizeof(int32_t) * (size_t)g.cap * (size_t)t->b;
+    build_lds_kernel<LAYOUT><<<blocks, kBuildLdsBlock, lds, st>>>(
+        t->batch_first.as<int32_t>(), t->nbatches, t->d_off, t->rows.as<int32_t>(), t->d_elems,
+        t->lengths.as<int32_t>(), t->toff.as<int64_t>(), t->BN, t->b, t->M, g.cap, T, t->d_status(), conf);
   }
+  if (t->nlarge > 0) {
+    int blocks = device_cu_count() * 4;
+    if (blocks > t->nlarge) blocks = t->nlarge;
+    fill_large_kernel<<<blocks, kBuildBlock, 0, st>>>(t->large_list.as<int32_t>(), t->nlarge,
+                                                      t->lengths.as<int32_t>(), t->toff.as<int64_t>(),
+                                                      t->BN, t->b, T);
+    insert_large_kernel<LAYOUT><<<blocks, kBuildBlock, 0, st>>>(
+        t->large_list.as<int32_t>(), t->nlarge, t->d_off, t->d_elems, t->lengths.as<int32_t>(),
+        t->toff.as<int64_t>(), t->BN, t->b, t->M, T, t->d_status(), conf);
+  }
+}
+
+// a4 + a6 for the layouts in `mask` (tables allocated, batches planned).
+// Conflicts are added to the counters at status+16 (normal) and status+24
+// (hierarchical).
+static rs_status fill_and_insert(rs_tables* t, int32_t mask, hipStream_t st) {
+  auto* conf = reinterpret_cast<unsigned long long*>(t->status.as<char>() + 16);
+  if (mask & RS_BUILD_NORMAL) build_layout<RS_NORMAL>(t, t->t_normal.as<int32_t>(), conf, st);
+  if (mask & RS_BUILD_HIERARCHICAL) build_layout<RS_HIERARCHICAL>(t, t->t_hier.as<int32_t>(), conf + 1, st);
   RS_HIP(hipGetLastError());
   return RS_OK;
 }
@@ -216,6 +406,7 @@ rs_status rs_tables_build(const int32_t* d_offsets, int64_t num_sets, const int3
   const size_t tbytes = sizeof(int32_t) * (size_t)t->BN * (size_t)bucket_size;
   if ((layouts & RS_BUILD_NORMAL) && (s = t->t_normal.alloc(tbytes)) != RS_OK) return fail(s);
   if ((layouts & RS_BUILD_HIERARCHICAL) && (s = t->t_hier.alloc(tbytes)) != RS_OK) return fail(s);
+  if ((s = plan_batches(t, st)) != RS_OK) return fail(s);
   if ((s = fill_and_insert(t, layouts, st)) != RS_OK) return fail(s);
   int32_t err = 0;
   unsigned long long conf[2] = {0, 0};

@@ -7,6 +7,15 @@
 
 struct rs_tc_plan;
 
+namespace rs {
+// Geometry of the LDS table build: cap buckets per LDS plane, tables of up
+// to lsmall buckets are built in LDS, batches start in windows of wwin.
+struct BuildGeom {
+  int32_t cap = 0, lsmall = 0, wwin = 0;
+};
+BuildGeom build_geom(int32_t b);
+}  // namespace rs
+
 struct rs_tables {
   int device = 0;
   int64_t n = 0, E = 0, BN = 0;
@@ -20,6 +29,10 @@ struct rs_tables {
   rs::DevBuf t_normal;   // int32[BN*b]
   rs::DevBuf t_hier;     // int32[BN*b]
   rs::DevBuf status;     // int32 error bits, then 2 x uint64 conflict counters
+  rs::BuildGeom geom;          // the build's batches: first table of each
+  rs::DevBuf batch_first;      // int32[nbatches + 1]
+  rs::DevBuf large_list;       // int32[nlarge], tables built in HBM
+  int32_t nbatches = 0, nlarge = 0;
   int64_t conflicts[2] = {0, 0};
   rs_tc_plan* tc_plan = nullptr;  // cached by rs_tc_count
   // rs_tc_count holds it from the plan's creation to the count's end: the

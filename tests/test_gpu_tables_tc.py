@@ -104,6 +104,34 @@ def test_rebuild_in_place_keeps_invariants(api, oracle_lib):
     t1.close()
 
 
+@pytest.mark.parametrize("b", [1, 5, 16, 64])
+def test_build_in_lds_batches_single_tables_and_hbm(api, oracle_lib, b):
+    """The build takes three routes by table length: runs of small tables
+    share one LDS image, a table of up to a full image is built in LDS by
+    itself, and a longer one is filled and inserted in HBM.  Sets sized for
+    all three (at this b), empty sets among them, keep the oracle's table
+    invariants in both layouts, also after an in-place rebuild."""
+    rng = np.random.default_rng(100 + b)
+    sizes = np.concatenate([rng.integers(0, 200, 1500), rng.integers(1100, 2000, 12),
+                            rng.integers(2100, 6000, 6), [0, 0, 4500]])
+    rng.shuffle(sizes)
+    rows = [np.sort(rng.choice(1 << 24, int(n), replace=False)) for n in sizes]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    cols = np.concatenate(rows).astype(np.int32)
+    L, toff, M = oracle_lib.geometry(off, 0.2, b)
+    t = api.HashTables(_cuda(off), _cuda(cols), 0.2, b, 3)
+    assert np.array_equal(t.table_offsets().cpu().numpy(), toff)
+    for rebuild in (False, True):
+        if rebuild:
+            t.rebuild_async(3)
+            torch.cuda.synchronize()
+            t.status()
+        for layout in (0, 1):
+            ref = oracle_lib.Tables(L, toff, M, b, layout, None, 0)
+            assert oracle_lib.table_check(off, cols, ref, T=t.table(layout).cpu().numpy()) == 0
+    t.close()
+
+
 def test_table_overflow_is_reported(api):
     off = np.array([0, 200], np.int32)
     cols = np.arange(200, dtype=np.int32)
