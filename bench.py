@@ -327,7 +327,8 @@ def bench_tc(args, scale=None):
         bms = e0.elapsed_time(e1) / reps
         bbytes = 8 * d.num_elements + 4 * d.bucket_size * d.bucket_num + 12 * d.num_sets
         build[name] = {"ms": bms, "alg_bytes": bbytes, "GBps": bbytes / (bms / 1e3) / 1e9,
-                       "frac": bbytes / (bms / 1e3) / 1e9 / HBM_PEAK_GBS}
+                       "frac": bbytes / (bms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                       "traffic": _tc_build_traffic(scale, bbytes)}
     # the rebuilt tables count the same
     t.tc_count_async(out_t, 1, keys, False)
     _sync_all()
@@ -419,6 +420,15 @@ def _tc_traffic(scale, alg_bytes):
     tr = _traffic(f"tc_rmat{scale}")
     if tr and tr.get("alg_bytes_per_launch") == alg_bytes:
         return tr["hbm_bytes_per_launch"]
+    return None
+
+
+def _tc_build_traffic(scale, alg_bytes):
+    """Counter-measured HBM bytes of one layout's build, when profiled."""
+    tr = _traffic(f"tc_rmat{scale}")
+    b = tr.get("build", {}) if tr else {}
+    if b.get("alg_bytes_per_layout") == alg_bytes:
+        return b.get("hbm_bytes_per_layout")
     return None
 
 
