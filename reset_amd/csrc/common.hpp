@@ -106,6 +106,11 @@ __device__ __forceinline__ int probe(const int32_t* __restrict__ Tt, int64_t BN,
   return -1;
 }
 
+// An int32 in LDS, addressed as such: offsets into a table typed this way stay
+// 32-bit (a generic pointer's arithmetic is 64-bit until the address space is
+// inferred, which left a 64-bit multiply-add per bucket address).
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
 // ---------------------------------------------------------------- wave helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(kTcBlock) tc_tiled_kernel(
 // EMPTY slot; bounded at L*b slots.  The U keys advance in lockstep; a group
 // of lanes' keys that has resolved skips its step.
 template <int LAYOUT, int U>
-__device__ __forceinline__ uint32_t probe_lds(const int32_t* __restrict__ tab, uint32_t L,
+__device__ __forceinline__ uint32_t probe_lds(const lds_i32* __restrict__ tab, uint32_t L,
                                               int32_t b, uint32_t M, const int32_t (&x)[U],
                                               const bool (&act)[U], bool& bad) {
   uint32_t j[U], pos[U];
@@ -429,9 +429,9 @@ __device__ __forceinline__ uint32_t probe_lds(const int32_t* __restrict__ tab, u
 // All U keys take their home bucket together; the rare full bucket is
 // followed in a bounded loop (L buckets in all).
 template <int B>
-__device__ __forceinline__ void lds_bucket(const int32_t* __restrict__ tab, uint32_t j, int32_t x,
+__device__ __forceinline__ void lds_bucket(const lds_i32* __restrict__ tab, uint32_t j, int32_t x,
                                            bool& hit, bool& full) {
-  const int32_t* bk = tab + j * (uint32_t)B;
+  const lds_i32* bk = tab + __umul24(j, (uint32_t)B);  // (24-bit multiply: L <= 4096)
   uint32_t m = ~0u;
   int32_t last = 0;
 #pragma unroll
@@ -445,7 +445,7 @@ __device__ __forceinline__ void lds_bucket(const int32_t* __restrict__ tab, uint
 }
 
 template <int LAYOUT, int U, int B>
-__device__ __forceinline__ uint32_t probe_lds_buckets(const int32_t* __restrict__ tab, uint32_t L,
+__device__ __forceinline__ uint32_t probe_lds_buckets(const lds_i32* __restrict__ tab, uint32_t L,
                                                       uint32_t M, const int32_t (&x)[U],
                                                       const bool (&act)[U], bool& bad) {
   uint32_t j[U];
@@ -504,6 +504,7 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
     unsigned long long* __restrict__ unit_next) {
   extern __shared__ __attribute__((aligned(16))) int32_t lds_tc[];
   int32_t* const tab = lds_tc;
+  const lds_i32* const ltab = (const lds_i32*)lds_tc;  // the same table, 32-bit offsets
   const int lane = __lane_id();
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   int32_t* const marks = lds_tc + table_words + wid * (kWave * U);
@@ -593,9 +594,9 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
             x[uu] = keys[act[uu] ? p + d : 0];
           }
           if constexpr (B == 0)
-            found += probe_lds<LAYOUT, U>(tab, L, b, M, x, act, bad);
+            found += probe_lds<LAYOUT, U>(ltab, L, b, M, x, act, bad);
           else
-            found += probe_lds_buckets<LAYOUT, U, B>(tab, L, M, x, act, bad);
+            found += probe_lds_buckets<LAYOUT, U, B>(ltab, L, M, x, act, bad);
         }
         if (__popcll(__ballot(valid)) < kWave) break;
         e += kWave;
