@@ -132,6 +132,30 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Wave-wide max steps through DPP (gfx9 row shifts and row broadcasts): lanes
+// with no source lane keep `ident`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int32_t dpp_from(int32_t x, int32_t ident) {
+  return __builtin_amdgcn_update_dpp(ident, x, CTRL, ROWS, 0xf, false);
+}
+
+// Inclusive prefix max over the lanes of the wave (lane order).
+__device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
+  constexpr int32_t I = INT32_MIN;
+  x = max(x, dpp_from<0x111, 0xf>(x, I));  // row_shr:1
+  x = max(x, dpp_from<0x112, 0xf>(x, I));  // row_shr:2
+  x = max(x, dpp_from<0x114, 0xf>(x, I));  // row_shr:4
+  x = max(x, dpp_from<0x118, 0xf>(x, I));  // row_shr:8
+  x = max(x, dpp_from<0x142, 0xa>(x, I));  // row_bcast:15 into rows 1 and 3
+  x = max(x, dpp_from<0x143, 0xc>(x, I));  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
+// The value of lane - 1 (lane 0 gets `ident`): DPP wave_shr:1.
+__device__ __forceinline__ int32_t wave_shr1(int32_t x, int32_t ident) {
+  return dpp_from<0x138, 0xf>(x, ident);
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
