@@ -575,23 +575,45 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
         const int32_t S = __shfl(incl, kWave - 1, kWave);
         const int32_t delta = kb - excl;  // key index of probe p of this lane's edge: p + delta
         for (int32_t it = 0; it < S; it += kWave * U) {
-          ++tag;
-          if (valid && excl > it && excl < it + kWave * U) marks[excl - it] = tag;
+          const bool starts_in = valid && excl > it && excl < it + kWave * U;
           int32_t base = (int32_t)__popcll(__ballot(valid && excl <= it)) - 1;
           int32_t x[U];
           bool act[U];
+          if (!__any(starts_in)) {
+            // the slab lies inside one edge's probes (most of a long list's
+            // slabs): every probe's key index is p + that edge's delta
+            const int32_t d = __builtin_amdgcn_readlane(delta, base);
+            if (it + kWave * U <= S) {  // a full slab: keys from one uniform base
+              const int32_t* __restrict__ kp = keys + (int64_t)(it + d);
 #pragma unroll
-          for (int uu = 0; uu < U; ++uu) {
-            const bool m = marks[uu * kWave + lane] == tag;
-            const uint64_t Mu = __ballot(m);
-            const int32_t below = (int32_t)__builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(Mu >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)Mu, 0u));
-            const int32_t own = base + below + (m ? 1 : 0);
-            base += (int32_t)__popcll(Mu);
-            const int32_t p = it + uu * kWave + lane;
-            act[uu] = p < S;
-            const int32_t d = __shfl(delta, own & (kWave - 1), kWave);
-            x[uu] = keys[act[uu] ? p + d : 0];
+              for (int uu = 0; uu < U; ++uu) {
+                act[uu] = true;
+                x[uu] = kp[uu * kWave + lane];
+              }
+            } else {
+#pragma unroll
+              for (int uu = 0; uu < U; ++uu) {
+                const int32_t p = it + uu * kWave + lane;
+                act[uu] = p < S;
+                x[uu] = keys[act[uu] ? p + d : 0];
+              }
+            }
+          } else {
+            ++tag;
+            if (starts_in) marks[excl - it] = tag;
+#pragma unroll
+            for (int uu = 0; uu < U; ++uu) {
+              const bool m = marks[uu * kWave + lane] == tag;
+              const uint64_t Mu = __ballot(m);
+              const int32_t below = (int32_t)__builtin_amdgcn_mbcnt_hi(
+                  (uint32_t)(Mu >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)Mu, 0u));
+              const int32_t own = base + below + (m ? 1 : 0);
+              base += (int32_t)__popcll(Mu);
+              const int32_t p = it + uu * kWave + lane;
+              act[uu] = p < S;
+              const int32_t d = __shfl(delta, own & (kWave - 1), kWave);
+              x[uu] = keys[act[uu] ? p + d : 0];
+            }
           }
           if constexpr (B == 0)
             found += probe_lds<LAYOUT, U>(ltab, L, b, M, x, act, bad);
