@@ -151,6 +151,21 @@ __device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
   return x;
 }
 
+// Unsigned max and min over the whole wave (every lane gets it), by the same
+// DPP steps; the result is read from the last lane.
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t x) {
+  const int32_t I = MAX ? 0 : -1;
+  auto op = [](uint32_t a, uint32_t b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+  x = op(x, (uint32_t)dpp_from<0x111, 0xf>((int32_t)x, I));
+  x = op(x, (uint32_t)dpp_from<0x112, 0xf>((int32_t)x, I));
+  x = op(x, (uint32_t)dpp_from<0x114, 0xf>((int32_t)x, I));
+  x = op(x, (uint32_t)dpp_from<0x118, 0xf>((int32_t)x, I));
+  x = op(x, (uint32_t)dpp_from<0x142, 0xa>((int32_t)x, I));
+  x = op(x, (uint32_t)dpp_from<0x143, 0xc>((int32_t)x, I));
+  return (uint32_t)__builtin_amdgcn_readlane((int32_t)x, kWave - 1);
+}
+
 // The value of lane - 1 (lane 0 gets `ident`): DPP wave_shr:1.
 __device__ __forceinline__ int32_t wave_shr1(int32_t x, int32_t ident) {
   return dpp_from<0x138, 0xf>(x, ident);

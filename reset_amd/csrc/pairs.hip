@@ -518,12 +518,33 @@ __device__ __forceinline__ uint32_t pair_chunked_wave(const int32_t* __restrict_
     const int32_t staged_g = sc;
     const v4i staged = group_load(staged_g);          // unconditional, as above
     load_block<U>(G, pos + 2 * kBlock, 0, ng, ahead);  // unconditional, as above
+    bool done = false;
+    if (pos >= 0 && pos + kBlock <= ng) {
+      // a block wholly inside G: when its smallest and largest elements
+      // both lie in [lo, top] (most blocks: G is 8x denser than S), all of
+      // it is this chunk's, with no per-element routing or order check
+      uint32_t mn = (uint32_t)cur.x[0], mx = mn;
+#pragma unroll
+      for (int i = 1; i < 4 * U; ++i) {
+        mn = min(mn, (uint32_t)cur.x[i]);
+        mx = max(mx, (uint32_t)cur.x[i]);
+      }
+      if (wave_reduce_u32<false>(mn) >= lo && wave_reduce_u32<true>(mx) <= top) {
+        bool all[4 * U];
+#pragma unroll
+        for (int i = 0; i < 4 * U; ++i) all[i] = true;
+        hits += lds_lookup<B, 4 * U>(tab, L, displaced, cur.x, all, bad);
+        done = true;
+      }
+    }
     // an element below lo can only be met on the block's first pass: the
     // elements left for a later chunk lie above this chunk's top, which is
     // one below the next chunk's lo
+    if (!done) {
 #pragma unroll
-    for (int i = 0; i < 4 * U; ++i) unordered |= (uint32_t)cur.act[i] & (uint32_t)((uint32_t)cur.x[i] < lo);
-    while (true) {
+      for (int i = 0; i < 4 * U; ++i) unordered |= (uint32_t)cur.act[i] & (uint32_t)((uint32_t)cur.x[i] < lo);
+    }
+    while (!done) {
       bool in[4 * U];
       bool rest = false;
 #pragma unroll
