@@ -17,6 +17,7 @@
 // to the slots a sequential insert would give them, found by a prefix max
 // over the wave (pair_ordered_wave).  The per-pair count is reduced across
 // the wave and written once.
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -688,34 +689,46 @@ __device__ __forceinline__ uint32_t pair_ordered_wave(const int32_t* __restrict_
     const float span = (float)(last_el - first) + 1.0f;
     oh.scale = (float)nb * __builtin_amdgcn_rcpf(span);  // (any scale works: build and lookups share it)
     oh.bias = -(float)first * oh.scale;
-    const int32_t e0 = s0 + gk * kGroupElems + 4 * lane;  // element index of kv.x
+    const int32_t g0 = s0 + gk * kGroupElems;  // element index of the group's first slot
+    const int32_t e0 = g0 + 4 * lane;           // element index of kv.x
     const int32_t k[4] = {kv.x, kv.y, kv.z, kv.w};
-    bool act[4];
-    int32_t h[4], p[4];
+    // FULL: every element of the group is in the chunk (all chunks but a
+    // pair's first and last), so no element needs a mask
+    auto place = [&](auto full) {
+      constexpr bool FULL = decltype(full)::value;
+      bool act[4];
+      int32_t h[4], p[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      act[c] = e0 + c >= cb && e0 + c < ce;
-      h[c] = (int32_t)ordered_bucket((uint32_t)k[c], oh);
-      const int32_t t = act[c] ? B * h[c] - (4 * lane + c) : INT32_MIN / 2;
-      p[c] = c == 0 ? t : max(p[c - 1], t);
-    }
-    const int32_t before = wave_shr1(wave_prefix_max(p[3]), INT32_MIN / 2);
-    int32_t disp = 0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int32_t pos = 4 * lane + c + max(before, p[c]);
-      if (act[c]) {
-        tab[pos] = k[c];
-        disp = max(disp, pos / B - h[c]);
+      for (int c = 0; c < 4; ++c) {
+        act[c] = FULL || (e0 + c >= cb && e0 + c < ce);
+        h[c] = (int32_t)ordered_bucket((uint32_t)k[c], oh);
+        const int32_t t = act[c] ? B * h[c] - (4 * lane + c) : INT32_MIN / 2;
+        p[c] = c == 0 ? t : max(p[c - 1], t);
       }
-    }
-    D = __any(disp > 0) ? __builtin_amdgcn_readlane(wave_prefix_max(disp), kWave - 1) : 0;
-    // strictly ascending keys, above the previous chunk's range
-    const uint32_t prev = (uint32_t)wave_shr1(kv.w, 0);
-    uint32_t bad_order = (uint32_t)(act[0] && e0 - 1 >= cb && (uint32_t)k[0] <= prev);
+      const int32_t before = wave_shr1(wave_prefix_max(p[3]), INT32_MIN / 2);
+      int32_t disp = 0;
 #pragma unroll
-    for (int c = 1; c < 4; ++c) bad_order |= (uint32_t)(act[c] && act[c - 1] && (uint32_t)k[c] <= (uint32_t)k[c - 1]);
-    unordered |= bad_order | (uint32_t)(first < lo);
+      for (int c = 0; c < 4; ++c) {
+        const int32_t pos = 4 * lane + c + max(before, p[c]);
+        if (act[c]) {
+          tab[pos] = k[c];
+          disp = max(disp, (int32_t)((uint32_t)pos / B) - h[c]);
+        }
+      }
+      D = __any(disp > 0) ? __builtin_amdgcn_readlane(wave_prefix_max(disp), kWave - 1) : 0;
+      // strictly ascending keys, above the previous chunk's range
+      const uint32_t prev = (uint32_t)wave_shr1(kv.w, 0);
+      uint32_t bad_order = (uint32_t)(act[0] && e0 - 1 >= cb && (uint32_t)k[0] <= prev);
+#pragma unroll
+      for (int c = 1; c < 4; ++c)
+        bad_order |= (uint32_t)(act[c] && act[c - 1] && (uint32_t)k[c] <= (uint32_t)k[c - 1]);
+      unordered |= bad_order;
+    };
+    if (g0 >= 0 && ce - g0 == kGroupElems)
+      place(std::true_type{});
+    else
+      place(std::false_type{});
+    unordered |= (uint32_t)(first < lo);
     __builtin_amdgcn_wave_barrier();
     top = last ? 0xFFFFFFFFu : last_el;
   };
