@@ -1,5 +1,7 @@
 """GPU parity at the benchmark configurations' full sizes, against the CPU
-oracle: R-MAT scale-24 triangle counting (both layouts, against the
+oracle: the 1M-pair batch (every size equal to the overlap the generator
+planted, and a leading sample equal to the oracle's hash count), R-MAT
+scale-24 triangle counting (both layouts, against the
 sorted-merge count of the whole graph), the 1M-set similarity self-join
 (every planted near-duplicate found; the join restricted to a 200k-set
 prefix equal to the oracle's prefix join of that prefix) and all 20k
@@ -28,6 +30,28 @@ def api():
 
 def _say(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def test_pairs_config2_full_batch(api, oracle_lib):
+    """All 1,000,000 pairs of the headline batch (sizes log-uniform in
+    [16, 65536], 2^24 universe, overlap U[0, 0.5]) in one batched call with
+    the default per-bin settings: every size equals the overlap k the
+    generator planted, and the first 20,000 equal the oracle's hash count."""
+    from reset_amd import inputs
+
+    P = 1_000_000
+    batch = inputs.make_pair_batch(P, seed=SEED)
+    _say(f"pairs: {int(batch['a_off'][-1]) + int(batch['b_off'][-1])} elements")
+    plan = api.PairsPlan(P)
+    sizes = plan.run(batch["a_off"], batch["a"], batch["b_off"], batch["b"])
+    plan.close()
+    assert torch.equal(sizes, batch["k"])
+    S = 20_000
+    ao = batch["a_off"][:S + 1].cpu().numpy()
+    bo = batch["b_off"][:S + 1].cpu().numpy()
+    a = batch["a"][:int(ao[-1])].cpu().numpy()
+    b = batch["b"][:int(bo[-1])].cpu().numpy()
+    assert np.array_equal(sizes[:S].cpu().numpy(), oracle_lib.pairs_hash(ao, a, bo, b))
 
 
 def test_tc_rmat24_matches_oracle(api, oracle_lib):
