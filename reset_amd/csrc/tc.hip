@@ -61,6 +61,7 @@ namespace rs {
 
 constexpr int kTcBlock = 256;
 constexpr int64_t kTcTile = 4096;
+constexpr int64_t kTcTileMin = 512;
 // A table is staged when its probes reach kStageRatio x its L*b slots.
 constexpr int64_t kStageRatio = 2;
 // Units hold at least max(kUnitProbes, kUnitSlotFactor * L*b) probes (a table
@@ -965,13 +966,19 @@ static rs_status tc_count_layout(const rs_tables* t, const rs_tc_plan* p, const 
     RS_TRY((launch_staged_b<LAYOUT, kClassWaves[2]>(t, p, 2, T, keys, out, st)));
   const int64_t w_lo = p->wb[3], w_hi = p->wb[4];
   if (w_hi > w_lo) {
-    const int64_t ntiles = (w_hi - w_lo + kTcTile - 1) / kTcTile;
+    // tiles of kTcTile probes, smaller when that would leave waves of the
+    // grid without a tile (a small graph's few unstaged probes are chains
+    // of dependent HBM reads: spread them over the whole device)
+    const int64_t waves = (int64_t)device_cu_count() * 8 * (kTcBlock / kWave);
+    int64_t tile = kTcTile;
+    while (tile > kTcTileMin && (w_hi - w_lo) / tile < waves) tile >>= 1;
+    const int64_t ntiles = (w_hi - w_lo + tile - 1) / tile;
     int grid = grid_for(ntiles * kWave, kTcBlock, 8);
     grid = (grid + kXcds - 1) / kXcds * kXcds;
     tc_tiled_kernel<LAYOUT, 4><<<grid, kTcBlock, 0, st>>>(
         p->es.as<int32_t>(), p->eg.as<int32_t>(), t->d_off, keys, t->lengths.as<int32_t>(),
         t->toff.as<int64_t>(), T, t->BN, t->b, (uint32_t)t->M, p->W.as<int64_t>(), p->bounds[3],
-        p->bounds[4], w_lo, w_hi, kTcTile, out, t->d_status());
+        p->bounds[4], w_lo, w_hi, tile, out, t->d_status());
   }
   return RS_OK;
 }
