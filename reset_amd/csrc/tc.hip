@@ -429,20 +429,33 @@ __device__ __forceinline__ uint32_t probe_lds(const lds_i32* __restrict__ tab, u
 // absent; else the next bucket.  The B compares fold into one xor/min chain.
 // All U keys take their home bucket together; the rare full bucket is
 // followed in a bounded loop (L buckets in all).
+// With b = 5 the staged copy keeps slots 0-3 of bucket j as one 16-byte
+// group at tab[4j] and slot 4 at tab[4L + j] (split5 below): two LDS reads
+// per bucket instead of three for a 20-byte bucket.
 template <int B>
-__device__ __forceinline__ void lds_bucket(const lds_i32* __restrict__ tab, uint32_t j, int32_t x,
-                                           bool& hit, bool& full) {
-  const lds_i32* bk = tab + __umul24(j, (uint32_t)B);  // (24-bit multiply: L <= 4096)
-  uint32_t m = ~0u;
-  int32_t last = 0;
+__device__ __forceinline__ void lds_bucket(const lds_i32* __restrict__ tab, uint32_t L, uint32_t j,
+                                           int32_t x, bool& hit, bool& full) {
+  if constexpr (B == 5) {
+    typedef int v4i_l __attribute__((ext_vector_type(4)));
+    const v4i_l q = *reinterpret_cast<const __attribute__((address_space(3))) v4i_l*>(tab + 4u * j);
+    const int32_t v4 = tab[4u * L + j];
+    uint32_t m = min(min((uint32_t)(q.x ^ x), (uint32_t)(q.y ^ x)), min((uint32_t)(q.z ^ x), (uint32_t)(q.w ^ x)));
+    m = min(m, (uint32_t)(v4 ^ x));
+    hit = m == 0u;
+    full = v4 != kEmpty;
+  } else {
+    const lds_i32* bk = tab + __umul24(j, (uint32_t)B);  // (24-bit multiply: L <= 4096)
+    uint32_t m = ~0u;
+    int32_t last = 0;
 #pragma unroll
-  for (int k = 0; k < B; ++k) {
-    const int32_t v = bk[k];
-    m = min(m, (uint32_t)(v ^ x));
-    last = v;
+    for (int k = 0; k < B; ++k) {
+      const int32_t v = bk[k];
+      m = min(m, (uint32_t)(v ^ x));
+      last = v;
+    }
+    hit = m == 0u;
+    full = last != kEmpty;
   }
-  hit = m == 0u;
-  full = last != kEmpty;
 }
 
 template <int LAYOUT, int U, int B>
@@ -457,7 +470,7 @@ __device__ __forceinline__ uint32_t probe_lds_buckets(const lds_i32* __restrict_
   for (int u = 0; u < U; ++u) {
     j[u] = home_bucket<LAYOUT>((uint32_t)x[u], L, M);
     bool hit, full;
-    lds_bucket<B>(tab, j[u], x[u], hit, full);
+    lds_bucket<B>(tab, L, j[u], x[u], hit, full);
     hits += act[u] && hit;
     pend[u] = act[u] && !hit && full;
     any_pend |= pend[u];
@@ -470,7 +483,7 @@ __device__ __forceinline__ uint32_t probe_lds_buckets(const lds_i32* __restrict_
       for (uint32_t step = 1; step < L && __any(p); ++step) {
         jj = (jj + 1u) & (L - 1u);
         bool hit, full;
-        lds_bucket<B>(tab, jj, x[u], hit, full);
+        lds_bucket<B>(tab, L, jj, x[u], hit, full);
         hits += p && hit;
         p = p && !hit && full;
       }
@@ -542,6 +555,15 @@ __global__ void __launch_bounds__(NW * kWave) tc_staged_kernel(
       const int4 v = *reinterpret_cast<const int4*>(Tt + (int64_t)k * BN + jj);
       if constexpr (B == 0) {  // slot planes kept
         *reinterpret_cast<int4*>(tab + k * L + jj) = v;
+      } else if constexpr (B == 5) {  // slots 0-3 bucket-major, slot 4 in its own run
+        if (k < 4) {
+          tab[(jj + 0) * 4 + k] = v.x;
+          tab[(jj + 1) * 4 + k] = v.y;
+          tab[(jj + 2) * 4 + k] = v.z;
+          tab[(jj + 3) * 4 + k] = v.w;
+        } else {
+          *reinterpret_cast<int4*>(tab + 4 * L + jj) = v;
+        }
       } else {  // bucket-major
         tab[(jj + 0) * B + k] = v.x;
         tab[(jj + 1) * B + k] = v.y;
