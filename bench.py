@@ -536,6 +536,22 @@ def _ir_probe_work(t, q, qoff, layout):
     return total
 
 
+def _ir_hyper(args, off):
+    """(load factor, bucket size) of the IR tables: the reference solver's
+    least-cost pair (rs_hyper_solve) for the IR kernels' probe width -- one
+    lane per key -- and 128-byte transactions, at the index's mean posting
+    length; --ir-alpha / --ir-bucket override it."""
+    from reset_amd import api
+
+    n, E = off.numel() - 1, int(off[-1].item())
+    alpha, b, _ = api.hyper_solve(max(1, E // max(1, n)), W=1, S_trans=128)
+    if args.ir_alpha is not None:
+        alpha = args.ir_alpha
+    if args.ir_bucket is not None:
+        b = args.ir_bucket
+    return alpha, b
+
+
 def bench_ir_batch(args):
     """All IR queries of the 10M-document index in one batch, hierarchical
     tables, K times: the roofline of the query kernels alone (the command the
@@ -546,7 +562,8 @@ def bench_ir_batch(args):
     from reset_amd import api, inputs
 
     off, docs = inputs.ir_index(seed=SEED + 5)
-    t = api.HashTables(off, docs, 0.2, 5, 2)
+    alpha, b = _ir_hyper(args, off)
+    t = api.HashTables(off, docs, alpha, b, 2)
     q, qoff = inputs.ir_queries(off, args.queries, seed=SEED + 6)
     qd, qod = torch.from_numpy(q).cuda(), torch.from_numpy(qoff).cuda()
     for _ in range(args.warmup):
@@ -567,7 +584,8 @@ def bench_ir_batch(args):
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
             "data": "synthetic Zipf(1.1) inverted index (seeded on device)",
             "config": {"workload": f"{len(qoff) - 1} conjunctive queries of 2-8 terms over 10M docs / 1M terms, "
-                                   "one batch, hierarchical tables", "layout": "hierarchical"},
+                                   "one batch, hierarchical tables", "layout": "hierarchical",
+                       "load_factor": alpha, "bucket_size": b},
             "results": int(cnt.sum().item()),
             "roofline": {"bound": "hbm", "achieved": alg / (ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, "kernel_ms": ms,
@@ -625,7 +643,8 @@ def bench_ir(args):
 
     t0 = time.time()
     off, docs = inputs.ir_index(seed=SEED + 5)
-    t = api.HashTables(off, docs, 0.2, 5, 3)
+    alpha, b = _ir_hyper(args, off)
+    t = api.HashTables(off, docs, alpha, b, 3)
     q, qoff = inputs.ir_queries(off, args.queries, seed=SEED + 6)
     _sync_all()
     prep_s = time.time() - t0
@@ -651,6 +670,21 @@ def bench_ir(args):
         if lay == layout:
             resh, cnth, roffh = res.cpu().numpy(), cnt.cpu().numpy(), roff.cpu().numpy()
     t.set_timing(False)
+    # the same batch over tables with the reference apps' defaults (alpha
+    # 0.2, b 5; hierarchical): same results, for comparison
+    ref_hp = None
+    if (alpha, b) != (0.2, 5):
+        tr5 = api.HashTables(off, docs, 0.2, 5, 2)
+        tr5.ir_query(qd, qod, layout)
+        tr5.set_timing(True)
+        kms = []
+        for _ in range(3):
+            res5, cnt5, _ = tr5.ir_query(qd, qod, layout)
+            kms.append(tr5.kernel_ms())
+        same5 = bool(np.array_equal(cnt5.cpu().numpy(), cnth) and np.array_equal(res5.cpu().numpy(), resh))
+        tr5.close()
+        del res5, cnt5
+        ref_hp = {"load_factor": 0.2, "bucket_size": 5, "kernel_ms": float(np.mean(kms)), "same_results": same5}
     n0 = int(roffh[-1])
     probes = _ir_probe_work(t, q, qoff, layout)
     alg_bytes = 8 * n0 + 8 * probes
@@ -678,10 +712,13 @@ def bench_ir(args):
         "config": {"workload": "IR serving: 10M docs, 1M terms, posting lengths Zipf(1.1) capped at 2M, "
                                f"{Q} conjunctive queries of 2-8 terms, Poisson arrivals",
                    "postings": int(off[-1].item()), "queries": Q, "qps": args.qps, "layout": "hierarchical",
+                   "load_factor": alpha, "bucket_size": b,
+                   "hyperparameters": "rs_hyper_solve at probe width 1 (one lane per key), 128-byte transactions",
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"], "max": head["max_ms"]},
         "qps_sweep": sweep,
         "batch_all_queries": batch,
+        "batch_reference_hyperparameters": ref_hp,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "ir_segment_kernel with its count, scan and copy kernels (all queries in one batch)",
@@ -691,7 +728,8 @@ def bench_ir(args):
         "prep_s": prep_s,
     }
     tr = _traffic("ir_batch")
-    if tr and tr.get("alg_bytes_per_launch") == alg_bytes:
+    if tr and tr.get("alg_bytes_per_launch") == alg_bytes and \
+            (tr.get("load_factor"), tr.get("bucket_size")) == (alpha, b):
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
     if not args.no_cpu:
         import oracle
@@ -1029,6 +1067,9 @@ def main(argv=None):
                     help="ir: queries served per rate = rate x this (at most --queries, at least 2000)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="tc/ir: CPU baseline sample grown to this")
     ap.add_argument("--ir-budget-mb", type=int, default=4096)
+    ap.add_argument("--ir-alpha", type=float, default=None,
+                    help="ir: table load factor (default: the solver's pick for one lane per key)")
+    ap.add_argument("--ir-bucket", type=int, default=None, help="ir: table bucket size (default: the solver's)")
     ap.add_argument("--ir-batch", type=int, default=256)
     ap.add_argument("--mixed-inflight", type=int, default=4,
                     help="IR streams of the mixed serving run (the lookups take one more)")

@@ -109,6 +109,15 @@ def test_ir_config5_all_queries_match_merge(api, oracle_lib):
     # order); as sets per query they equal the merge's ascending lists
     qid = np.repeat(np.arange(len(m_cnt)), m_cnt)
     assert np.array_equal(got_h[np.lexsort((got_h, qid))], want)
+    t.close()
+    # the tables of the IR benchmark: the solver's (load factor, bucket size)
+    # for one lane per key
+    alpha, b, _ = api.hyper_solve(int(off[-1]) // (len(off) - 1), W=1, S_trans=128)
+    _say(f"ir: batch queries, hierarchical layout, tables alpha {alpha} b {b}")
+    t = api.HashTables(off_d, docs_d, alpha, b, 3)
+    res_b, cnt_b, _ = t.ir_query(qd, qod, 1)
+    assert np.array_equal(cnt_b.cpu().numpy(), m_cnt)
+    assert np.array_equal(_dense(res_b.cpu().numpy(), m_cnt, m_roff), want)
     _say("ir: all 20k queries through the serving loop")
     srv = serving.Server(t, budget_bytes=4096 << 20, max_batch=256, max_inflight=4)
     for i in range(len(qoff) - 1):

@@ -104,11 +104,15 @@ def _ir_case(rng, nterms=300, ndocs=200000, maxlen=20000, nq=400, kmax=6):
     return idx_off, docs, q, qoff
 
 
-def test_ir_query_matches_oracle_bit_exact(api, oracle_lib):
+@pytest.mark.parametrize("alpha,b", [(0.2, 5), (0.1, 1), (0.2, 2)])
+def test_ir_query_matches_oracle_bit_exact(api, oracle_lib, alpha, b):
+    """Both layouts against the oracle's hash path over the same tables and
+    the chained merge; (0.1, 1) is the solver's pick for one lane per key,
+    the IR benchmark's tables."""
     rng = np.random.default_rng(8)
     idx_off, docs, q, qoff = _ir_case(rng)
-    t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 3)
-    ot = oracle_lib.build_tables(idx_off, docs, 0.2, 5, 0)
+    t = api.HashTables(_cuda(idx_off), _cuda(docs), alpha, b, 3)
+    ot = oracle_lib.build_tables(idx_off, docs, alpha, b, 0)
     o_res, o_cnt, o_roff = oracle_lib.ir_hash(idx_off, docs, ot, q, qoff)
     m_res, m_cnt, _ = oracle_lib.ir_merge(idx_off, docs, q, qoff)
     res, cnt, roff = t.ir_query(_cuda(q), _cuda(qoff), 0)
