@@ -6,6 +6,7 @@ TC.cu:62-66 behave)."""
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -199,3 +200,13 @@ def test_intersection_cli_usage(built_tools):
     r = subprocess.run([os.path.join(built_tools, "intersection"), "5"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so,
+    before touching a device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "exactly one GPU" in (r.stdout + r.stderr)

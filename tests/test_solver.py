@@ -118,3 +118,12 @@ def test_full_grid_matches_reference_csv(sc):
         assert abs(api.hyper_cost(m, 0.9, 4, 32, 128, 4) - dict(((a, b), c) for a, b, c in rows)[(0.9, 4)]) > 2e-6
     # the recommendation of the reference's summary (res/*_summary.json)
     assert api.hyper_solve(n, 32, 128, 4)[:2] == (0.2, 5)
+
+
+def test_one_lane_probes_pick_one_slot_buckets():
+    """The IR query kernels probe with one lane per key (W = 1). At 128-byte
+    transactions the model's least-cost pair is then load factor 0.1 with
+    one-slot buckets at every posting length of the IR index: the pair the
+    IR benchmark builds its tables with."""
+    for n in (2, 68, 4096, 2_000_000):
+        assert api.hyper_solve(n, 1, 128, 4)[:2] == (0.1, 1)
