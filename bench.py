@@ -932,6 +932,13 @@ def bench_compare(args):
     ms = _timed(lambda: res.__setitem__("v", t.ir_query(qd, qod, 1, term0_docs=ikeys)), steps, warm)
     ir["reset_hierarchical"] = {"query_ms": ms, "results": int(res["v"][1].sum().item())}
     t.close()
+    # the same load factor with one-slot buckets (the IR kernels' probe width
+    # is one lane per key): a probe's walk stays in one slot plane
+    t = api.HashTables(ioff, idocs, 0.2, 1, 2)
+    ikeys = t.probe_order(1)
+    ms = _timed(lambda: res.__setitem__("v", t.ir_query(qd, qod, 1, term0_docs=ikeys)), steps, warm)
+    ir["reset_hierarchical_b1"] = {"query_ms": ms, "results": int(res["v"][1].sum().item())}
+    t.close()
     for k in kinds:
         c = api.CompetitorTables(ioff, idocs, k, 0.2)
         ms = _timed(lambda: res.__setitem__("v", c.ir_query(qd, qod)), steps, warm)
