@@ -149,15 +149,17 @@ def test_ir_long_term0_lists_multi_segment(api, oracle_lib):
     t.close()
 
 
+@pytest.mark.parametrize("alpha,b", [(0.2, 5), (0.1, 1)])
 @pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("steal_min", [0, 1, 7])
-def test_ir_work_stealing_schedule_matches_merge(api, oracle_lib, layout, steal_min):
+def test_ir_work_stealing_schedule_matches_merge(api, oracle_lib, layout, steal_min, alpha, b):
     """The work-stealing segment schedule (forced on small batches through
     the steal threshold) returns the chained merge's lists, ascending, for
-    both layouts with term0 read from the index."""
+    both layouts with term0 read from the index, on 5-slot and on one-slot
+    tables."""
     rng = np.random.default_rng(30 + steal_min)
     idx_off, docs, q, qoff = _ir_case(rng, nterms=200, ndocs=400000, maxlen=40000, nq=300, kmax=8)
-    t = api.HashTables(_cuda(idx_off), _cuda(docs), 0.2, 5, 3)
+    t = api.HashTables(_cuda(idx_off), _cuda(docs), alpha, b, 3)
     m_res, m_cnt, m_roff = oracle_lib.ir_merge(idx_off, docs, q, qoff)
     try:
         api.ir_set_steal_threshold(steal_min)
